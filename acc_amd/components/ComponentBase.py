@@ -1,0 +1,88 @@
+"""Component plugin API (acc/components/ComponentBase.py).
+
+As in the reference, a concrete component class (name not ending in "Base")
+must define `propagate`, and its constructor sets `self.name` and
+`self.propagate_params` (same tuple contents as the reference's).  The
+reference's `propagate` is a Numba device function; here it is a
+`DeviceFunction`: the name of the gfx950 device routine the fuser inlines into
+the fused kernel, plus how the component packs its parameters for the C ABI
+(include/mcv_acc.h).  Subclasses implement `_mcv_desc()`.
+
+`process(neutrons)` is the buffered mode of the reference
+(ComponentBase.py:79-107): an (N, 10) float64 array in, the surviving rows
+(p > 0) out.
+"""
+import numpy as np
+
+
+class DeviceFunction:
+    """Marker for a component's device routine.
+
+    `call` is a format string producing the C++ statement(s) for component
+    index {i}; the fused kernel exposes `rng`, `n` (the neutron), `P` (packed
+    parameters), `H` (histograms) and the launch block `A`.
+    """
+
+    def __init__(self, call, stochastic=False, monitor=False, preamble=None):
+        self.call = call
+        self.stochastic = stochastic
+        self.monitor = monitor
+        self.preamble = preamble
+
+    def render(self, comp, i):
+        return self.call.format(i=i, **comp._render_context(i))
+
+    def render_preamble(self, comp, i):
+        if self.preamble is None:
+            return ""
+        return self.preamble(comp, i)
+
+
+class Curator(type):
+    """Registers concrete components; requires `propagate`
+    (acc/components/ComponentBase.py:21-32)."""
+
+    components = []
+
+    def __new__(cls, name, bases, dct):
+        x = super().__new__(cls, name, bases, dct)
+        if not name.endswith("Base"):
+            if not hasattr(x, "propagate") or not isinstance(x.propagate, DeviceFunction):
+                raise TypeError(f"{name} should define `propagate` method")
+            Curator.components.append(x)
+        return x
+
+
+class ComponentBase(metaclass=Curator):
+
+    propagate_params = ()
+
+    # flag whether this is a multiple scattering component
+    is_multiplescattering = False
+    NUM_MULTIPLE_SCATTER = 10
+
+    mcv_type = None
+
+    def _render_context(self, i):
+        return {}
+
+    def _mcv_desc(self):
+        raise NotImplementedError
+
+    def mcv_desc(self):
+        """dict(type, params, iparams, data, nbins) for include/mcv_acc.h's mcv_comp_desc."""
+        d = self._mcv_desc()
+        d.setdefault("iparams", np.zeros(0, dtype=np.int64))
+        d.setdefault("data", np.zeros(0))
+        d.setdefault("nbins", 0)
+        return d
+
+    def process(self, neutrons, seed=None):
+        """Buffered mode: propagate an (N, 10) array through this component and
+        return the rows with p > 0 (acc/components/ComponentBase.py:79-107)."""
+        from ..buffered import process_component
+        return process_component(self, neutrons, seed=seed)
+
+
+class StochasticComponentBase(ComponentBase):
+    """Components drawing random numbers (acc/components/StochasticComponentBase.py)."""

@@ -1,0 +1,20 @@
+"""Launch defaults (acc/config.py:1-4).
+
+`rng_seed` is read at run time, not bound at import as the reference does
+(acc/components/sources/SourceBase.py:18), and `run()` also takes an explicit
+`seed`.  `gate_dead_neutrons` selects the histogram semantics: True drops a
+neutron once its weight p <= 0 (the mcvine engine's, and the reference's
+buffered mode, acc/components/ComponentBase.py:219-221); False reproduces the
+reference fused kernel, where an absorbed neutron keeps flowing downstream
+(acc/run_script.py:182-196, acc/neutron.py:48-58).
+"""
+ntotalthreads = int(1e6)
+threads_per_block = 512
+floattype = "float64"
+rng_seed = 1
+gate_dead_neutrons = True
+
+
+def get_numpy_floattype():
+    import numpy as np
+    return getattr(np, floattype)

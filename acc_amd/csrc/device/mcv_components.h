@@ -1,0 +1,342 @@
+// mcv_components.h -- per-component device code of the fused kernels.
+//
+// Each function is the gfx950 form of a reference component's `propagate`
+// (cited per function).  Parameters arrive as a pointer into the packed
+// parameter blob of the instrument (layouts in include/mcv_acc.h); all lanes
+// read the same addresses, so they compile to scalar loads.  Arithmetic keeps
+// the reference's operation order; FMA contraction is left to the compiler.
+#pragma once
+#include "mcv_core.h"
+
+namespace mcv {
+
+// ------------------------------------------------------------------ Source_simple
+// acc/components/sources/source_simple.py:114-161 (+ randvec_target_rect :17-42).
+// P[0..12] = reference params; P[13..15] = p1, P[16..18] = p2 (the two unit
+// vectors randvec_target_rect derives from the target, computed on the host
+// with the reference's formulas); P[19] = length(target).
+__device__ __forceinline__ void source_simple(Rng& rng, Neutron& n, const double* __restrict__ P) {
+  const double r1 = rng.uniform(), r2 = rng.uniform(), r3 = rng.uniform(), r4 = rng.uniform(), r5 = rng.uniform();
+  double x, y;
+  if (P[0] != 0.0) {
+    x = P[1] * (r1 - 0.5);
+    y = P[2] * (r2 - 0.5);
+  } else {
+    const double chi = (2 * PI) * r1;
+    // numba types sqrt(float32) as float32: the correctly rounded float sqrt,
+    // obtained exactly as float(sqrt_double) (double rounding is innocuous for
+    // sqrt since 53 >= 2*24 + 2)
+    const double r = (double)(float)sqrt(r2) * P[3];
+    double s, c;
+    sincos(chi, &s, &c);
+    x = r * c;
+    y = r * s;
+  }
+  const double xw = P[9], yh = P[10], dist = P[11];
+  // randvec_target_rect(target=(0,0,dist), xw, yh, r3, r4)
+  const double dx0 = xw * (r3 * 2 - 1) / 2.0;
+  const double dy0 = yh * (r4 * 2 - 1) / 2.0;
+  const double L = P[19];
+  const double vf0 = P[16] * dy0 + (P[13] * dx0 + 0.0);
+  const double vf1 = P[17] * dy0 + (P[14] * dx0 + 0.0);
+  const double vf2 = P[18] * dy0 + (P[15] * dx0 + dist);
+  const double dist2 = sqrt(dx0 * dx0 + dy0 * dy0 + L * L);
+  const double solidangle = (xw * yh * L) / (dist2 * dist2 * dist2);
+  (void)vf2;
+  const double dx = vf0 - x, dy = vf1 - y;
+  const double dist1 = sqrt(dx * dx + dy * dy + dist * dist);
+  double v;
+  if (P[4] != 0.0) {
+    const double Lw = P[5] + P[6] * (r5 * 2 - 1);
+    v = K2V * ((2 * PI) / Lw);
+  } else {
+    const double E = P[7] + P[8] * (r5 * 2 - 1);
+    v = SE2V * sqrt(E);
+  }
+  n.x = x;
+  n.y = y;
+  n.z = 0.0;
+  n.vx = v * dx / dist1;
+  n.vy = v * dy / dist1;
+  n.vz = v * dist / dist1;
+  n.t = 0.0;
+  n.p = P[12] * solidangle;
+}
+
+// ------------------------------------------------------------------ supermirror
+// acc/components/optics/_guide_utils.py:6-23
+__device__ __forceinline__ double calc_reflectivity(double Q, double R0, double Qc, double alpha, double m,
+                                                    double W) {
+  double R = R0;
+  if (Q > Qc) {
+    const double tmp = (Q - m * Qc) / W;
+    if (tmp < 10)
+      R *= (1 - tanh(tmp)) * (1 - alpha * (Q - Qc)) / 2;
+    else
+      R = 0;
+  }
+  return R;
+}
+
+// ------------------------------------------------------------------ Guide
+// acc/components/optics/guide.py:59-162.  P = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W
+__device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) {
+  const double ww = P[0], hh = P[1], hw1 = P[2], hh1 = P[3], l = P[4];
+  double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz, t = n.t, prob = n.p;
+  const double dt = -z / vz;
+  x += vx * dt;
+  y += vy * dt;
+  z = 0.;
+  t += dt;
+  if (x <= -hw1 || x >= hw1 || y <= -hh1 || y >= hh1) {
+    n.p = -1;  // absorb(): the rest of the state is left as it was
+    return;
+  }
+  for (int nb = 0; nb < 100000; ++nb) {
+    const double av = l * vx, bv = ww * vz, ah = l * vy, bh = hh * vz;
+    const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
+    const double cv1 = -hw1 * l - z * ww, cv2 = x * l, ch1 = -hh1 * l - z * hh, ch2 = y * l;
+    double t1 = (l - z) / vz;
+    int i = 0;
+    if (vdotn_v1 < 0) {
+      const double t2 = (cv1 - cv2) / vdotn_v1;
+      if (t2 < t1) { t1 = t2; i = 1; }
+    }
+    if (vdotn_v2 < 0) {
+      const double t2 = (cv1 + cv2) / vdotn_v2;
+      if (t2 < t1) { t1 = t2; i = 2; }
+    }
+    if (vdotn_h1 < 0) {
+      const double t2 = (ch1 - ch2) / vdotn_h1;
+      if (t2 < t1) { t1 = t2; i = 3; }
+    }
+    if (vdotn_h2 < 0) {
+      const double t2 = (ch1 + ch2) / vdotn_h2;
+      if (t2 < t1) { t1 = t2; i = 4; }
+    }
+    if (i == 0) break;
+    x += vx * t1;
+    y += vy * t1;
+    z += vz * t1;
+    t += t1;
+    // reflection off mirror i: the four cases share one form with (axis, sign)
+    const bool vert = i <= 2;
+    const double wh = vert ? ww : hh;
+    const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
+    const double nlen2 = l * l + wh * wh;
+    const double q = V2K * (-2) * vdn / sqrt(nlen2);
+    const double d = 2 * vdn / nlen2;
+    const double dl = d * l;
+    if (i == 1) vx = vx - dl;
+    else if (i == 2) vx = vx + dl;
+    else if (i == 3) vy = vy - dl;
+    else vy = vy + dl;
+    vz = vz - d * wh;
+    const double R = calc_reflectivity(q, P[5], P[6], P[7], P[8], P[9]);
+    prob *= R;
+    if (prob <= 0) break;
+  }
+  n.x = x; n.y = y; n.z = z; n.vx = vx; n.vy = vy; n.vz = vz;
+  n.t = t;
+  n.p = prob;
+}
+
+// ------------------------------------------------------------------ Guide_tapering
+// acc/components/optics/guide_tapering.py:141-291.
+// P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
+// D = 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf).
+__device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restrict__ P,
+                                               const double* __restrict__ D) {
+  const double l_seg = P[0], R0 = P[1], W = P[8];
+  const int nseg = (int)P[9];
+  const double* __restrict__ w1 = D;
+  const double* __restrict__ h1 = D + 2 * nseg;
+  const double* __restrict__ wwa = D + 4 * nseg;
+  const double* __restrict__ hha = D + 5 * nseg;
+  const double* __restrict__ whalf = D + 6 * nseg;
+  const double* __restrict__ hhalf = D + 7 * nseg;
+  double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz, t = n.t, prob = n.p;
+  double dt = -z / vz;
+  x += vx * dt;
+  y += vy * dt;
+  z = 0.0;
+  t += dt;
+  for (int seg = 0; seg < nseg; ++seg) {
+    const double zr = (double)seg * l_seg;
+    const double w1s = w1[seg], hhs = hhalf[seg], wws = wwa[seg], hhh = hha[seg], whs = whalf[seg];
+    dt = (zr - z) / vz;
+    x += vx * dt;
+    y += vy * dt;
+    z += vz * dt;
+    if (x <= -0.5 * w1s || x >= 0.5 * w1s || y <= -hhs || y >= hhs) {
+      n.p = 0;
+      return;
+    }
+    x += 0.5 * w1s;
+    const double edge = floor(x / w1s) * w1s;
+    if (x - edge > w1s) {
+      n.p = 0;
+      return;
+    }
+    x -= (edge + 0.5 * w1s);
+    const double hadj = edge + (0.5 * w1s) - 0.5 * w1s;
+    for (int nb = 0; nb < 100000; ++nb) {
+      const double av = l_seg * vx, bv = wws * vz, ah = l_seg * vy, bh = hhh * vz;
+      const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
+      const double cv1 = -whs * l_seg - (z - zr) * wws, cv2 = x * l_seg;
+      const double ch1 = -hhs * l_seg - (z - zr) * hhh, ch2 = y * l_seg;
+      double t1 = (zr + l_seg - z) / vz;
+      int i = 0;
+      if (vdotn_v1 < 0) {
+        const double t2 = (cv1 - cv2) / vdotn_v1;
+        if (t2 < t1) { t1 = t2; i = 1; }
+      }
+      if (vdotn_v2 < 0) {
+        const double t2 = (cv1 + cv2) / vdotn_v2;
+        if (t2 < t1) { t1 = t2; i = 2; }
+      }
+      if (vdotn_h1 < 0) {
+        const double t2 = (ch1 - ch2) / vdotn_h1;
+        if (t2 < t1) { t1 = t2; i = 3; }
+      }
+      if (vdotn_h2 < 0) {
+        const double t2 = (ch1 + ch2) / vdotn_h2;
+        if (t2 < t1) { t1 = t2; i = 4; }
+      }
+      if (i == 0) break;
+      x += vx * t1;
+      y += vy * t1;
+      z += vz * t1;
+      t += t1;
+      const bool vert = i <= 2;
+      const double wh = vert ? wws : hhh;
+      const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
+      const double nlen2 = l_seg * l_seg + wh * wh;
+      const double q = V2K * (-2.0) * vdn / sqrt(nlen2);
+      const double d = 2.0 * vdn / nlen2;
+      const double dl = d * l_seg;
+      if (i == 1) vx = vx - dl;
+      else if (i == 2) vx = vx + dl;
+      else if (i == 3) vy = vy - dl;
+      else vy = vy + dl;
+      vz = vz - d * wh;
+      const double mxy = vert ? P[6] : P[7];
+      if (mxy == 0) {
+        x += hadj;
+        break;
+      }
+      double m = mxy;
+      if (m <= 0.0) m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
+      const double R = calc_reflectivity(q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W);
+      if (R > 1e-10)
+        prob *= R;
+      else {
+        n.p = 0;
+        return;
+      }
+    }
+    x += hadj;
+  }
+  n.x = x; n.y = y; n.z = z; n.vx = vx; n.vy = vy; n.vz = vz;
+  n.t = t;
+  n.p = prob;
+}
+
+// ------------------------------------------------------------------ monitors
+// acc/neutron.py:83-89 prop_z0 (monitors do not write it back)
+__device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, double& t) {
+  const double dt = -n.z / n.vz;
+  x = n.x + n.vx * dt;
+  y = n.y + n.vy * dt;
+  t = n.t + dt;
+}
+
+// acc/components/monitors/psd_monitor.py:79-96.  P = xmin, xmax, ymin, ymax, nx, ny
+__device__ __forceinline__ void psd_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+  double x, y, t;
+  prop_z0(n, x, y, t);
+  if (n.t > t) return;
+  const double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3];
+  // strict window as the reference; written so that NaN is rejected too
+  if (!(x > xmin && x < xmax && y > ymin && y < ymax)) return;
+  const int64_t nx = (int64_t)P[4], ny = (int64_t)P[5];
+  int64_t ix = (int64_t)floor((x - xmin) / (xmax - xmin) * (double)nx);
+  int64_t iy = (int64_t)floor((y - ymin) / (ymax - ymin) * (double)ny);
+  ix = ix < 0 ? 0 : (ix < nx ? ix : nx - 1);
+  iy = iy < 0 ? 0 : (iy < ny ? iy : ny - 1);
+  hist_add3(H, nx * ny, ix * ny + iy, n.p);
+}
+
+// acc/components/monitors/wavelength_monitor.py:149-171.  P = xmin..ymax, Lmin, Lmax, nchan
+__device__ __forceinline__ void wavelength_monitor(const Neutron& n, const double* __restrict__ P,
+                                                   double* __restrict__ H) {
+  double x, y, t;
+  prop_z0(n, x, y, t);
+  if (n.t > t) return;
+  if (!(x > P[0] && x < P[1] && y > P[2] && y < P[3])) return;
+  const double v = sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz);
+  const double L = 2 * PI / (v * V2K);
+  const double Lmin = P[4], Lmax = P[5];
+  if (!(L > Lmin && L < Lmax)) return;
+  const int64_t nchan = (int64_t)P[6];
+  int64_t iL = (int64_t)floor((L - Lmin) / (Lmax - Lmin) * (double)nchan);
+  iL = iL < 0 ? 0 : (iL < nchan ? iL : nchan - 1);
+  hist_add3(H, nchan, iL, n.p);
+}
+
+// acc/components/monitors/iqe_monitor.py:251-284.
+// P = Ei, Qmin, Qmax, Emin, Emax, max_in, min_in, max_out, min_out, nQ, nE
+__device__ __forceinline__ void iqe_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+  const double vx = n.vx, vy = n.vy, vz = n.vz;
+  const double angle_in_plane = atan2(vx, vz) / PI * 180.;
+  double angle_out_of_plane = 0.;
+  if (vy != 0) {
+    const double theta = atan(sqrt(vx * vx + vz * vz) / vy) / PI * 180.;
+    angle_out_of_plane = vy > 0 ? 90. - theta : 90. + theta;
+  }
+  if (!(P[6] < angle_in_plane && angle_in_plane < P[5] && P[8] < angle_out_of_plane &&
+        angle_out_of_plane < P[7]))
+    return;
+  const double Ei = P[0], Qmin = P[1], Qmax = P[2], Emin = P[3], Emax = P[4];
+  const double vi = sqrt(Ei) * SE2V;
+  const double Q = sqrt(vx * vx + vy * vy + (vz - vi) * (vz - vi)) * V2K;
+  const double E = Ei - (vx * vx + vy * vy + vz * vz) * VS2E;
+  if (!((Q >= Qmin && Q < Qmax) && (E >= Emin && E < Emax))) return;
+  const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
+  int64_t iQ = (int64_t)floor((Q - Qmin) / (Qmax - Qmin) * (double)nQ);
+  int64_t iE = (int64_t)floor((E - Emin) / (Emax - Emin) * (double)nE);
+  iQ = iQ < 0 ? 0 : (iQ < nQ ? iQ : nQ - 1);
+  iE = iE < 0 ? 0 : (iE < nE ? iE : nE - 1);
+  hist_add3(H, nQ * nE, iQ * nE + iE, n.p);
+}
+
+// Cylindrical array of vertical tubes (no reference counterpart; monitor
+// conventions of psd_monitor.py).  P = radius, height, phi_min, phi_max (deg),
+// tof_min, tof_max (s), tube_diameter, unused, ntubes, npixels, ntof.
+__device__ __forceinline__ void tube_detector(const Neutron& n, const double* __restrict__ P,
+                                              double* __restrict__ H) {
+  const double R = P[0], Ht = P[1], phimin = P[2], phimax = P[3], tmin = P[4], tmax = P[5];
+  const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+  const double a = vx * vx + vz * vz;
+  if (a == 0) return;
+  const double b = x * vx + z * vz, c = x * x + z * z - R * R;
+  const double k = b * b - a * c;
+  if (k < 0) return;
+  const double dt = (-b + sqrt(k)) / a;
+  if (dt < 0) return;
+  const double xh = x + vx * dt, yh = y + vy * dt, zh = z + vz * dt, tof = n.t + dt;
+  const double phi = atan2(xh, zh) / PI * 180.;
+  if (!(phi >= phimin && phi < phimax)) return;
+  if (!(yh > -Ht / 2 && yh < Ht / 2)) return;
+  if (!(tof >= tmin && tof < tmax)) return;
+  const int64_t ntubes = (int64_t)P[8], npix = (int64_t)P[9], ntof = (int64_t)P[10];
+  int64_t it = (int64_t)floor((phi - phimin) / (phimax - phimin) * (double)ntubes);
+  int64_t ip = (int64_t)floor((yh + Ht / 2) / Ht * (double)npix);
+  int64_t ib = (int64_t)floor((tof - tmin) / (tmax - tmin) * (double)ntof);
+  it = it < 0 ? 0 : (it < ntubes ? it : ntubes - 1);
+  ip = ip < 0 ? 0 : (ip < npix ? ip : npix - 1);
+  ib = ib < 0 ? 0 : (ib < ntof ? ib : ntof - 1);
+  hist_add3(H, ntubes * npix * ntof, (it * npix + ip) * ntof + ib, n.p);
+}
+
+}  // namespace mcv

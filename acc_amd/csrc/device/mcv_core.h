@@ -1,0 +1,73 @@
+// mcv_core.h -- device primitives of the fused neutron ray-tracing kernels (gfx950).
+//
+// Neutron state lives in registers as ten doubles (the layout of
+// acc/neutron.py: x, y, z, vx, vy, vz, s1, s2, t, p).  The random stream is
+// numba's xoroshiro128+ (numba/cuda/random.py, numba 0.53.1) with the state
+// held in four VGPRs instead of being re-read from global memory per draw.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcv {
+
+// mcni.utils.conversion (McStas values); K2V = 1/V2K.
+constexpr double V2K = 1.58825361e-3;
+constexpr double SE2V = 437.393377;
+constexpr double VS2E = 5.22703725e-6;
+constexpr double K2V = 1.0 / 1.58825361e-3;
+constexpr double PI = 3.141592653589793;
+
+struct Neutron {
+  double x, y, z, vx, vy, vz, s1, s2, t, p;
+};
+
+// ------------------------------------------------------------------ RNG
+struct Rng {
+  uint64_t s0, s1;
+
+  __device__ __forceinline__ uint64_t next() {
+    const uint64_t a = s0, b0 = s1;
+    const uint64_t result = a + b0;
+    const uint64_t b = b0 ^ a;
+    s0 = ((a << 55) | (a >> 9)) ^ b ^ (b << 14);
+    s1 = (b << 36) | (b >> 28);
+    return result;
+  }
+  // uint64_to_unit_float32: float32(float64(x >> 11) * 2^-53); may round to 1.0f
+  __device__ __forceinline__ double uniform() {
+    const uint64_t x = next() >> 11;
+    const double d = (double)x * 0x1p-53;
+    return (double)(float)d;
+  }
+};
+
+// ------------------------------------------------------------------ frames
+// acc/neutron.py:41-46 (abs2rel): r' = R (r - o); v' = R v.  T = {R[9], o[3]}.
+__device__ __forceinline__ void apply_transformation(Neutron& n, const double* __restrict__ T) {
+  const double rx = n.x - T[9], ry = n.y - T[10], rz = n.z - T[11];
+  const double vx = n.vx, vy = n.vy, vz = n.vz;
+  n.x = T[0] * rx + T[1] * ry + T[2] * rz;
+  n.y = T[3] * rx + T[4] * ry + T[5] * rz;
+  n.z = T[6] * rx + T[7] * ry + T[8] * rz;
+  n.vx = T[0] * vx + T[1] * vy + T[2] * vz;
+  n.vy = T[3] * vx + T[4] * vy + T[5] * vz;
+  n.vz = T[6] * vx + T[7] * vy + T[8] * vz;
+}
+
+__device__ __forceinline__ void prop_dt_inplace(Neutron& n, double dt) {
+  n.x = n.x + n.vx * dt;
+  n.y = n.y + n.vy * dt;
+  n.z = n.z + n.vz * dt;
+  n.t = n.t + dt;
+}
+
+// ------------------------------------------------------------------ histograms
+// Monitors own (3, nbins) float64 blocks: N, sum p, sum p^2.  Device memory
+// from hipMalloc is coarse-grained, so the FP64 add executes in L2.
+__device__ __forceinline__ void hist_add3(double* __restrict__ H, int64_t nbins, int64_t b, double p) {
+  unsafeAtomicAdd(H + b, 1.0);
+  unsafeAtomicAdd(H + nbins + b, p);
+  unsafeAtomicAdd(H + 2 * nbins + b, p * p);
+}
+
+}  // namespace mcv

@@ -1,0 +1,32 @@
+// mcv_launch.h -- argument block of a fused instrument kernel, shared by the
+// host runtime (mcv_runtime.hip) and the generated chain kernels.
+#pragma once
+#include <stdint.h>
+
+#define MCV_MAX_COMP 32
+
+namespace mcv {
+
+// Passed by value as the single kernel argument; every field is uniform, so
+// the kernel keeps it in SGPRs.  Offsets index the packed parameter blob P
+// (per-component parameters, data tables and the (R, o) frame changes) and the
+// histogram buffer H.
+struct LaunchArgs {
+  const double* P;
+  double* H;
+  const uint64_t* rng;               // (nthreads, 2) stream states
+  unsigned long long* counter;       // processed neutrons
+  double* dump;                      // (dump_count, 10) final states, or null
+  int32_t* dump_alive;               // (dump_count)
+  int64_t dump_first, dump_count;
+  int64_t N, nthreads, npt;          // reference partition
+  int64_t tid_base;                  // first stream of this launch
+  int32_t gate;                      // MCV_FLAG_GATE_DEAD
+  int32_t ncomp;
+  int64_t off[MCV_MAX_COMP];         // parameters of component i
+  int64_t doff[MCV_MAX_COMP];        // data table of component i
+  int64_t hoff[MCV_MAX_COMP];        // (3, nbins) histogram of monitor i
+  int64_t xoff[MCV_MAX_COMP];        // frame change i -> i+1: R[9], o[3]
+};
+
+}  // namespace mcv

@@ -1,0 +1,594 @@
+// mcv_runtime.hip -- host runtime behind include/mcv_acc.h (libmcvacc.so).
+//
+// Owns device memory, the xoroshiro128+ stream seeding, module loading of the
+// fused chain kernels the fuser compiled, launches and histogram read-back.
+// Parameter packing keeps each component's reference parameter order and
+// appends the few derived constants the device code wants (documented in
+// mcv_components.h / mcv_samples.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mcv_acc.h"
+#include "device/mcv_launch.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(MCV_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ stream seeding
+// numba's create_xoroshiro128p_states (numba/cuda/random.py, numba 0.53.1):
+// stream 0 = SplitMix64(seed) in both words, stream i = stream i-1 advanced by
+// the 2^64-step jump.  The jump is linear over GF(2), so stream i = J^i s0; the
+// device evaluates J^i as a product of precomputed J^(2^k) (one 128x128 bit
+// matrix per set bit of i) instead of the reference's sequential host loop.
+uint64_t splitmix64(uint64_t seed) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct V128 {
+  uint64_t a, b;
+};
+
+inline V128 xnext(V128 s) {
+  uint64_t s0 = s.a, s1 = s.b;
+  s1 ^= s0;
+  V128 r;
+  r.a = ((s0 << 55) | (s0 >> 9)) ^ s1 ^ (s1 << 14);
+  r.b = (s1 << 36) | (s1 >> 28);
+  return r;
+}
+
+V128 xjump(V128 s) {
+  static const uint64_t J[2] = {0xbeac0467eba5facbull, 0xd86b048b86aa9922ull};
+  uint64_t a = 0, b = 0;
+  for (int i = 0; i < 2; ++i)
+    for (int bit = 0; bit < 64; ++bit) {
+      if (J[i] & (1ull << bit)) {
+        a ^= s.a;
+        b ^= s.b;
+      }
+      s = xnext(s);
+    }
+  return V128{a, b};
+}
+
+// matrix = 128 columns (image of each basis bit: bits 0..63 of s0, then s1)
+struct Mat {
+  V128 col[128];
+};
+
+inline V128 apply(const Mat& M, V128 v) {
+  V128 o{0, 0};
+  for (int k = 0; k < 64; ++k)
+    if ((v.a >> k) & 1) {
+      o.a ^= M.col[k].a;
+      o.b ^= M.col[k].b;
+    }
+  for (int k = 0; k < 64; ++k)
+    if ((v.b >> k) & 1) {
+      o.a ^= M.col[64 + k].a;
+      o.b ^= M.col[64 + k].b;
+    }
+  return o;
+}
+
+constexpr int kJumpMats = 48;  // streams up to 2^48
+
+const std::vector<uint64_t>& jump_matrices() {
+  static std::vector<uint64_t> flat;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    Mat P;
+    for (int k = 0; k < 128; ++k) {
+      V128 e{k < 64 ? (1ull << k) : 0ull, k >= 64 ? (1ull << (k - 64)) : 0ull};
+      P.col[k] = xjump(e);
+    }
+    flat.resize((size_t)kJumpMats * 256);
+    for (int m = 0; m < kJumpMats; ++m) {
+      for (int k = 0; k < 128; ++k) {
+        flat[(size_t)m * 256 + 2 * k] = P.col[k].a;
+        flat[(size_t)m * 256 + 2 * k + 1] = P.col[k].b;
+      }
+      Mat Q;
+      for (int k = 0; k < 128; ++k) Q.col[k] = apply(P, P.col[k]);
+      P = Q;
+    }
+  });
+  return flat;
+}
+
+__global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restrict__ mats, int nmats, uint64_t z,
+                                                        int64_t n, uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t a = z, b = z;
+  for (int k = 0; k < nmats; ++k) {
+    if ((i >> k) & 1) {
+      const uint64_t* __restrict__ M = mats + (size_t)k * 256;
+      uint64_t oa = 0, ob = 0;
+#pragma unroll 8
+      for (int bit = 0; bit < 64; ++bit) {
+        const uint64_t m = 0ull - ((a >> bit) & 1ull);
+        oa ^= M[2 * bit] & m;
+        ob ^= M[2 * bit + 1] & m;
+      }
+#pragma unroll 8
+      for (int bit = 0; bit < 64; ++bit) {
+        const uint64_t m = 0ull - ((b >> bit) & 1ull);
+        oa ^= M[128 + 2 * bit] & m;
+        ob ^= M[128 + 2 * bit + 1] & m;
+      }
+      a = oa;
+      b = ob;
+    }
+  }
+  out[2 * i] = a;
+  out[2 * i + 1] = b;
+}
+
+struct DeviceCtx {
+  int device = -1;
+  uint64_t* d_mats = nullptr;
+};
+
+std::mutex g_ctx_mu;
+std::vector<DeviceCtx> g_ctx;
+
+int device_ctx(DeviceCtx** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ctx_mu);
+  for (auto& c : g_ctx)
+    if (c.device == dev) {
+      *out = &c;
+      return MCV_OK;
+    }
+  DeviceCtx c;
+  c.device = dev;
+  const auto& flat = jump_matrices();
+  HIPCHECK(hipMalloc(&c.d_mats, flat.size() * sizeof(uint64_t)));
+  HIPCHECK(hipMemcpy(c.d_mats, flat.data(), flat.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  g_ctx.push_back(c);
+  *out = &g_ctx.back();
+  return MCV_OK;
+}
+
+int nbits_for(int64_t n) {
+  int k = 0;
+  while (k < 63 && (int64_t(1) << k) < n) ++k;
+  return k;
+}
+
+int seed_streams(hipStream_t s, uint64_t seed, int64_t n, uint64_t* d_out) {
+  if (n <= 0) return MCV_OK;
+  DeviceCtx* ctx = nullptr;
+  int rc = device_ctx(&ctx);
+  if (rc) return rc;
+  int nm = nbits_for(n);
+  if (nm > kJumpMats) return fail(MCV_ERR_ARG, "too many streams (%lld)", (long long)n);
+  const int bs = 256;
+  const int64_t grid = (n + bs - 1) / bs;
+  hipLaunchKernelGGL(mcv_seed_streams, dim3((unsigned)grid), dim3(bs), 0, s, ctx->d_mats, nm, splitmix64(seed), n,
+                     d_out);
+  HIPCHECK(hipGetLastError());
+  return MCV_OK;
+}
+
+// ------------------------------------------------------------------ packing
+struct Packed {
+  std::vector<double> params;
+  std::vector<double> data;
+  int64_t nbins = 0;
+};
+
+void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+void normalize3(double* v) {
+  const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const double s = 1.0 / l;
+  v[0] *= s;
+  v[1] *= s;
+  v[2] *= s;
+}
+
+int expect(const mcv_comp_desc& d, int i, int np, int nip) {
+  if (d.n_params != np || (np && !d.params))
+    return fail(MCV_ERR_ARG, "component %d (type %d): expected %d params, got %d", i, d.type, np, d.n_params);
+  if (d.n_iparams != nip || (nip && !d.iparams))
+    return fail(MCV_ERR_ARG, "component %d (type %d): expected %d iparams, got %d", i, d.type, nip, d.n_iparams);
+  return MCV_OK;
+}
+
+int pack(const mcv_comp_desc& d, int i, Packed& out) {
+  auto copyp = [&](int np) { out.params.assign(d.params, d.params + np); };
+  int rc;
+  switch (d.type) {
+    case MCV_ARM:
+      return MCV_OK;
+    case MCV_SOURCE_SIMPLE: {
+      if ((rc = expect(d, i, 13, 0))) return rc;
+      copyp(13);
+      // randvec_target_rect's unit vectors (acc/components/sources/source_simple.py:129-142)
+      const double target[3] = {0.0, 0.0, d.params[11]}, vertical[3] = {0.0, 1.0, 0.0};
+      double p1[3], p2[3];
+      cross3(target, vertical, p1);
+      normalize3(p1);
+      cross3(target, p1, p2);
+      normalize3(p2);
+      for (int k = 0; k < 3; ++k) out.params.push_back(p1[k]);
+      for (int k = 0; k < 3; ++k) out.params.push_back(p2[k]);
+      out.params.push_back(std::sqrt(target[0] * target[0] + target[1] * target[1] + target[2] * target[2]));
+      return MCV_OK;
+    }
+    case MCV_GUIDE:
+      if ((rc = expect(d, i, 10, 0))) return rc;
+      copyp(10);
+      return MCV_OK;
+    case MCV_GUIDE_TAPERING: {
+      if ((rc = expect(d, i, 9, 1))) return rc;
+      const int64_t nseg = d.iparams[0];
+      if (nseg < 1 || d.n_data != 10 * nseg || !d.data)
+        return fail(MCV_ERR_ARG, "component %d: tapered guide needs a 10 x nseg table (nseg=%lld, n_data=%lld)", i,
+                    (long long)nseg, (long long)d.n_data);
+      copyp(9);
+      out.params.push_back((double)nseg);
+      out.data.assign(d.data, d.data + d.n_data);
+      return MCV_OK;
+    }
+    case MCV_SAMPLE_HSS:
+    case MCV_SAMPLE_HMS: {
+      if ((rc = expect(d, i, 16, 8))) return rc;
+      copyp(16);
+      for (int k = 0; k < 8; ++k) out.params.push_back((double)d.iparams[k]);
+      const int64_t nn = d.iparams[0];
+      if (nn < 1 || d.n_data < nn * MCV_CSG_NODE_SIZE || !d.data)
+        return fail(MCV_ERR_ARG, "component %d: sample shape table too small", i);
+      if (d.iparams[1] == MCV_KERNEL_SQE_GRID && d.iparams[5] + d.iparams[6] * d.iparams[7] > d.n_data)
+        return fail(MCV_ERR_ARG, "component %d: S(Q,E) grid exceeds the data block", i);
+      out.data.assign(d.data, d.data + d.n_data);
+      return MCV_OK;
+    }
+    case MCV_MON_PSD: {
+      if ((rc = expect(d, i, 4, 2))) return rc;
+      if (d.n_bins != d.iparams[0] * d.iparams[1] || d.n_bins <= 0)
+        return fail(MCV_ERR_ARG, "component %d: PSD n_bins != nx*ny", i);
+      copyp(4);
+      out.params.push_back((double)d.iparams[0]);
+      out.params.push_back((double)d.iparams[1]);
+      out.nbins = d.n_bins;
+      return MCV_OK;
+    }
+    case MCV_MON_WAVELENGTH: {
+      if ((rc = expect(d, i, 6, 1))) return rc;
+      if (d.n_bins != d.iparams[0] || d.n_bins <= 0)
+        return fail(MCV_ERR_ARG, "component %d: wavelength n_bins != nchan", i);
+      copyp(6);
+      out.params.push_back((double)d.iparams[0]);
+      out.nbins = d.n_bins;
+      return MCV_OK;
+    }
+    case MCV_MON_IQE: {
+      if ((rc = expect(d, i, 9, 2))) return rc;
+      if (d.n_bins != d.iparams[0] * d.iparams[1] || d.n_bins <= 0)
+        return fail(MCV_ERR_ARG, "component %d: IQE n_bins != nQ*nE", i);
+      copyp(9);
+      out.params.push_back((double)d.iparams[0]);
+      out.params.push_back((double)d.iparams[1]);
+      out.nbins = d.n_bins;
+      return MCV_OK;
+    }
+    case MCV_DET_TUBES: {
+      if ((rc = expect(d, i, 8, 3))) return rc;
+      if (d.n_bins != d.iparams[0] * d.iparams[1] * d.iparams[2] || d.n_bins <= 0)
+        return fail(MCV_ERR_ARG, "component %d: detector n_bins != ntubes*npixels*ntof", i);
+      copyp(8);
+      for (int k = 0; k < 3; ++k) out.params.push_back((double)d.iparams[k]);
+      out.nbins = d.n_bins;
+      return MCV_OK;
+    }
+  }
+  return fail(MCV_ERR_ARG, "component %d: unknown type %d", i, d.type);
+}
+
+int64_t align8(int64_t n) { return (n + 7) & ~int64_t(7); }
+
+}  // namespace
+
+struct mcv_instr {
+  int ncomp = 0;
+  std::vector<int> types;
+  std::vector<int> nparams;  // reference-order params per component (settable)
+  mcv::LaunchArgs args{};
+  std::vector<double> blob;
+  std::vector<int64_t> nbins;
+  int64_t hist_total = 0;
+  double* d_blob = nullptr;
+  double* d_hist = nullptr;
+  uint64_t* d_rng = nullptr;
+  int64_t rng_cap = 0;
+  unsigned long long* d_counter = nullptr;
+  double* d_dump = nullptr;
+  int32_t* d_alive = nullptr;
+  int64_t dump_first = 0, dump_count = 0;
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+void release(mcv_instr* h) {
+  if (!h) return;
+  if (h->d_blob) (void)hipFree(h->d_blob);
+  if (h->d_hist) (void)hipFree(h->d_hist);
+  if (h->d_rng) (void)hipFree(h->d_rng);
+  if (h->d_counter) (void)hipFree(h->d_counter);
+  if (h->d_dump) (void)hipFree(h->d_dump);
+  if (h->d_alive) (void)hipFree(h->d_alive);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->mod) (void)hipModuleUnload(h->mod);
+  delete h;
+}
+
+int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
+                const char* code_object, const char* kernel_name, mcv_instr** out) {
+  if (!comps || ncomp < 1) return fail(MCV_ERR_ARG, "empty component chain");
+  if (ncomp > MCV_MAX_COMP) return fail(MCV_ERR_ARG, "chain longer than %d components", MCV_MAX_COMP);
+  if (ncomp > 1 && (!offsets || !rotmats)) return fail(MCV_ERR_ARG, "missing frame transformations");
+  if (!code_object || !kernel_name) return fail(MCV_ERR_ARG, "missing code object / kernel name");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MCV_ERR_HIP, "no HIP device available");
+
+  mcv_instr* h = new mcv_instr();
+  *out = h;
+  h->ncomp = ncomp;
+  std::vector<Packed> packed(ncomp);
+  for (int i = 0; i < ncomp; ++i) {
+    int rc = pack(comps[i], i, packed[i]);
+    if (rc) return rc;
+    h->types.push_back(comps[i].type);
+    h->nparams.push_back(comps[i].n_params);
+  }
+  // blob: frame changes, then per-component parameters, then data tables
+  std::vector<double>& B = h->blob;
+  for (int i = 0; i + 1 < ncomp; ++i) {
+    h->args.xoff[i] = (int64_t)B.size();
+    for (int k = 0; k < 9; ++k) B.push_back(rotmats[9 * i + k]);
+    for (int k = 0; k < 3; ++k) B.push_back(offsets[3 * i + k]);
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  for (int i = 0; i < ncomp; ++i) {
+    h->args.off[i] = (int64_t)B.size();
+    B.insert(B.end(), packed[i].params.begin(), packed[i].params.end());
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  for (int i = 0; i < ncomp; ++i) {
+    h->args.doff[i] = (int64_t)B.size();
+    B.insert(B.end(), packed[i].data.begin(), packed[i].data.end());
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  if (B.empty()) B.resize(8, 0.0);
+  h->hist_total = 0;
+  for (int i = 0; i < ncomp; ++i) {
+    h->args.hoff[i] = h->hist_total;
+    h->nbins.push_back(packed[i].nbins);
+    h->hist_total += 3 * packed[i].nbins;
+  }
+  h->args.ncomp = ncomp;
+
+  HIPCHECK(hipStreamCreate(&h->stream));
+  for (auto& e : h->ev) HIPCHECK(hipEventCreate(&e));
+  HIPCHECK(hipMalloc(&h->d_blob, B.size() * sizeof(double)));
+  HIPCHECK(hipMemcpy(h->d_blob, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMalloc(&h->d_hist, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
+  HIPCHECK(hipMemset(h->d_hist, 0, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
+  HIPCHECK(hipMalloc(&h->d_counter, sizeof(unsigned long long)));
+
+  hipError_t e = hipModuleLoad(&h->mod, code_object);
+  if (e != hipSuccess) return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
+  e = hipModuleGetFunction(&h->fn, h->mod, kernel_name);
+  if (e != hipSuccess)
+    return fail(MCV_ERR_MODULE, "kernel %s not in %s: %s", kernel_name, code_object, hipGetErrorString(e));
+  return MCV_OK;
+}
+
+void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks, int64_t& nthreads, int64_t& npt) {
+  if (ntotalthreads <= 0) ntotalthreads = 1000000;  // acc/config.py:1
+  if (tpb <= 0) tpb = 512;                          // acc/config.py:2
+  const int64_t ntot = (int64_t)std::min<uint64_t>(N, (uint64_t)ntotalthreads);
+  nblocks = (ntot + tpb - 1) / tpb;
+  nthreads = nblocks * tpb;
+  npt = nthreads > 0 ? (int64_t)((N + (uint64_t)nthreads - 1) / (uint64_t)nthreads) : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcv_abi_version(void) { return MCV_ABI_VERSION; }
+
+const char* mcv_last_error(void) { return g_last_error.c_str(); }
+
+int mcv_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+mcv_instr* mcv_instr_create(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
+                            const char* code_object, const char* kernel_name) {
+  mcv_instr* h = nullptr;
+  int rc = create_impl(comps, ncomp, offsets, rotmats, code_object, kernel_name, &h);
+  if (rc) {
+    std::string msg = g_last_error;
+    release(h);
+    g_last_error = msg;
+    return nullptr;
+  }
+  return h;
+}
+
+int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32_t n_params) {
+  if (!h || comp < 0 || comp >= h->ncomp) return fail(MCV_ERR_ARG, "bad handle or component index");
+  if (n_params != h->nparams[comp] || (n_params && !params))
+    return fail(MCV_ERR_ARG, "component %d expects %d params", comp, h->nparams[comp]);
+  if (h->types[comp] == MCV_SOURCE_SIMPLE)
+    return fail(MCV_ERR_ARG, "Source_simple parameters are fixed at create time");
+  const int64_t off = h->args.off[comp];
+  std::memcpy(h->blob.data() + off, params, sizeof(double) * n_params);
+  HIPCHECK(hipMemcpy(h->d_blob + off, params, sizeof(double) * n_params, hipMemcpyHostToDevice));
+  return MCV_OK;
+}
+
+int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
+                  mcv_run_stats* stats) {
+  if (!h) return fail(MCV_ERR_ARG, "null handle");
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t nblocks, nthreads, npt;
+  partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->nblocks = nblocks;
+    stats->nthreads = nthreads;
+    stats->npt = npt;
+  }
+  if (ncount == 0) return MCV_OK;
+  if (nthreads > h->rng_cap) {
+    if (h->d_rng) HIPCHECK(hipFree(h->d_rng));
+    h->d_rng = nullptr;
+    HIPCHECK(hipMalloc(&h->d_rng, (size_t)nthreads * 2 * sizeof(uint64_t)));
+    h->rng_cap = nthreads;
+  }
+  HIPCHECK(hipMemsetAsync(h->d_counter, 0, sizeof(unsigned long long), h->stream));
+  HIPCHECK(hipEventRecord(h->ev[0], h->stream));
+  int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord(h->ev[1], h->stream));
+
+  mcv::LaunchArgs A = h->args;
+  A.P = h->d_blob;
+  A.H = h->d_hist;
+  A.rng = h->d_rng;
+  A.counter = h->d_counter;
+  A.dump = h->d_dump;
+  A.dump_alive = h->d_alive;
+  A.dump_first = h->dump_first;
+  A.dump_count = h->d_dump ? h->dump_count : 0;
+  A.N = (int64_t)ncount;
+  A.nthreads = nthreads;
+  A.npt = npt;
+  A.tid_base = 0;
+  A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+  const unsigned bs = 256;
+  const int64_t grid = (nthreads + bs - 1) / bs;
+  void* kargs[] = {&A};
+  HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, 0, h->stream, kargs, nullptr));
+  HIPCHECK(hipEventRecord(h->ev[2], h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  unsigned long long processed = 0;
+  HIPCHECK(hipMemcpy(&processed, h->d_counter, sizeof(processed), hipMemcpyDeviceToHost));
+  float ms_rng = 0, ms_k = 0;
+  HIPCHECK(hipEventElapsedTime(&ms_rng, h->ev[0], h->ev[1]));
+  HIPCHECK(hipEventElapsedTime(&ms_k, h->ev[1], h->ev[2]));
+  if (stats) {
+    stats->processed = processed;
+    stats->ms_rng_init = ms_rng;
+    stats->ms_kernel = ms_k;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (processed != ncount)
+    return fail(MCV_ERR_COUNT, "Processed neutron count %llu does not match requested neutron count %llu",
+                (unsigned long long)processed, (unsigned long long)ncount);
+  return MCV_OK;
+}
+
+int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3) {
+  if (!h || comp < 0 || comp >= h->ncomp) return fail(MCV_ERR_ARG, "bad handle or component index");
+  if (h->nbins[comp] <= 0) return fail(MCV_ERR_ARG, "component %d is not a monitor", comp);
+  HIPCHECK(hipMemcpy(out3, h->d_hist + h->args.hoff[comp], 3 * h->nbins[comp] * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return MCV_OK;
+}
+
+int mcv_monitor_clear(mcv_instr* h) {
+  if (!h) return fail(MCV_ERR_ARG, "null handle");
+  if (h->hist_total > 0) HIPCHECK(hipMemset(h->d_hist, 0, h->hist_total * sizeof(double)));
+  return MCV_OK;
+}
+
+int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count) {
+  if (!h || first < 0 || count < 0) return fail(MCV_ERR_ARG, "bad dump range");
+  if (h->d_dump) HIPCHECK(hipFree(h->d_dump));
+  if (h->d_alive) HIPCHECK(hipFree(h->d_alive));
+  h->d_dump = nullptr;
+  h->d_alive = nullptr;
+  h->dump_first = first;
+  h->dump_count = count;
+  if (count == 0) return MCV_OK;
+  HIPCHECK(hipMalloc(&h->d_dump, (size_t)count * 10 * sizeof(double)));
+  HIPCHECK(hipMalloc(&h->d_alive, (size_t)count * sizeof(int32_t)));
+  HIPCHECK(hipMemset(h->d_dump, 0, (size_t)count * 10 * sizeof(double)));
+  HIPCHECK(hipMemset(h->d_alive, 0xff, (size_t)count * sizeof(int32_t)));
+  return MCV_OK;
+}
+
+int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive) {
+  if (!h || !h->d_dump) return fail(MCV_ERR_ARG, "no dump configured");
+  HIPCHECK(hipMemcpy(states, h->d_dump, (size_t)h->dump_count * 10 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(alive, h->d_alive, (size_t)h->dump_count * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return MCV_OK;
+}
+
+int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out) {
+  if (n < 0 || (n > 0 && !out)) return fail(MCV_ERR_ARG, "bad stream count");
+  if (n == 0) return MCV_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MCV_ERR_HIP, "no HIP device available");
+  uint64_t* d = nullptr;
+  HIPCHECK(hipMalloc(&d, (size_t)n * 2 * sizeof(uint64_t)));
+  int rc = seed_streams(nullptr, seed, n, d);
+  if (rc == MCV_OK) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(MCV_ERR_HIP, "stream seeding: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+void mcv_instr_destroy(mcv_instr* h) { release(h); }
+
+}  // extern "C"

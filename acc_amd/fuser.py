@@ -1,0 +1,121 @@
+"""Instrument fuser: one component chain -> one gfx950 kernel.
+
+The reference text-generates a Python module that inlines every component's
+Numba `propagate` into one @cuda.jit kernel, with applyTransformation between
+components (acc/run_script.py:175-277).  This fuser emits the C++ body of the
+same chain (each component's device routine, frame changes between them, and a
+dead-neutron test when MCV_FLAG_GATE_DEAD is set), wraps it in the per-stream
+loop of mcv_kernel.h, and compiles it with hipcc into a code object cached by
+the hash of its source.  Only the chain's structure (component kinds, sample
+shapes and kernel kinds) enters the source: parameter values, tables and
+histogram sizes are device data bound at run time, so one code object serves
+any parameter set of that structure.
+"""
+import glob
+import hashlib
+import os
+import subprocess
+import tempfile
+
+_here = os.path.dirname(os.path.abspath(__file__))
+DEVICE_DIR = os.path.join(_here, "csrc", "device")
+CACHE_DIR = os.environ.get("MCV_KERNEL_CACHE", os.path.join(_here, "_kcache"))
+OFFLOAD_ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIPCC_FLAGS = [
+    "--genco", f"--offload-arch={OFFLOAD_ARCH}", "-O3", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _device_headers_digest():
+    h = hashlib.sha1()
+    for p in sorted(glob.glob(os.path.join(DEVICE_DIR, "*.h"))):
+        with open(p, "rb") as f:
+            h.update(os.path.basename(p).encode())
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def chain_body(components):
+    """C++ statements of the per-neutron chain body (plain chains)."""
+    lines = []
+    for i, comp in enumerate(components):
+        if i > 0:
+            lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
+        lines.append(comp.propagate.render(comp, i))
+        lines.append("if (A.gate && !(n.p > 0)) return false;")
+    lines.append("return true;")
+    return lines
+
+
+def generate_source(instrument):
+    """(source text, structural description) of the fused kernel."""
+    comps = instrument.components
+    if not comps:
+        raise ValueError("empty instrument")
+    from .components.ComponentBase import ComponentBase
+    for c in comps:
+        assert isinstance(c, ComponentBase), f"{c} is not an acc_amd component"
+    if any(c.is_multiplescattering for c in comps):
+        from .fuser_ms import generate_ms_source
+        return generate_ms_source(instrument)
+    structure = " -> ".join(type(c).__name__ for c in comps)
+    pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
+    body = "\n  ".join(chain_body(comps))
+    src = f"""// fused instrument kernel: {structure}
+#include "mcv_kernel.h"
+
+namespace {{
+{pre}
+__device__ __forceinline__ bool chain_body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)rng; (void)H;
+  {body}
+}}
+}}  // namespace
+
+MCV_DEFINE_FUSED_KERNEL(KERNEL_NAME, chain_body)
+"""
+    return src, structure
+
+
+def kernel_key(src):
+    h = hashlib.sha1()
+    h.update(src.encode())
+    h.update(" ".join(HIPCC_FLAGS).encode())
+    h.update(_device_headers_digest().encode())
+    return h.hexdigest()[:16]
+
+
+def compile_source(src, verbose=False):
+    """Compile a generated chain source; returns (code object path, kernel name)."""
+    key = kernel_key(src)
+    name = f"mcv_fused_{key}"
+    os.makedirs(CACHE_DIR, exist_ok=True)
+    out = os.path.join(CACHE_DIR, name + ".hsaco")
+    if os.path.exists(out):
+        return out, name
+    text = src.replace("KERNEL_NAME", name)
+    with tempfile.TemporaryDirectory() as td:
+        srcpath = os.path.join(td, name + ".hip")
+        with open(srcpath, "w") as f:
+            f.write(text)
+        tmp_out = out + f".tmp{os.getpid()}"
+        cmd = [HIPCC] + HIPCC_FLAGS + ["-I", DEVICE_DIR, srcpath, "-o", tmp_out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed for fused kernel %s:\n%s\n%s" % (name, " ".join(cmd), r.stderr[-4000:]))
+        os.replace(tmp_out, out)
+        with open(os.path.join(CACHE_DIR, name + ".hip"), "w") as f:
+            f.write(text)
+    if verbose:
+        print("compiled", out)
+    return out, name
+
+
+def fuse(instrument, verbose=False):
+    """Compile (or fetch from the cache) the fused kernel of an instrument."""
+    src, _ = generate_source(instrument)
+    return compile_source(src, verbose=verbose)

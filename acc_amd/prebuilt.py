@@ -1,0 +1,85 @@
+"""Instruments whose fused kernels are compiled ahead of time by the build step.
+
+Every chain structure the packaged workloads, the smoke run and the GPU tests
+use is listed here, so a machine that only runs (and does not compile) finds
+its code objects in the kernel cache.
+"""
+import os
+
+
+def _c2_no_taper():
+    from .instrument import instrument
+    from .configs import _source
+    from .components.optics.guide import Guide
+    from .components.monitors.wavelength_monitor import Wavelength_monitor
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("c2-straight")
+    i.append(_source(52.5, 47.5), position=(0, 0, 0))
+    i.append(Guide('guide', w1=0.05, h1=0.05, w2=0.05, h2=0.05, l=10.), position=(0, 0, 10.0))
+    i.append(Wavelength_monitor('wavelength', Lmin=1., Lmax=10., nchan=200, xwidth=0.06, yheight=0.06),
+             position=(0, 0, 20.001))
+    i.append(PSD_monitor('psd', xwidth=0.06, yheight=0.06, nx=100, ny=100), position=(0, 0, 20.001))
+    return i
+
+
+def _source_only():
+    from .instrument import instrument
+    from .configs import _source
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("source-psd")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.2, yheight=0.2, nx=40, ny=40), position=(0, 0, 10.0))
+    return i
+
+
+def _hollow_cylinder_hss():
+    from .instrument import instrument
+    from .configs import _source, isotropic_al_kernel
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .geometry import Cylinder, Difference
+    i = instrument("hollow-cylinder-hss")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    shape = Difference(Cylinder(0.015, 0.05), Cylinder(0.01, 0.06))
+    HSS = factory(shape=shape, kernel=isotropic_al_kernel())
+    s = HSS('sample')
+    i.append(s, position=(0, 0, 10.0), orientation=(90, 0, 0))
+    i.append(IQE_monitor('iqe', Ei=60., Qmin=0., Qmax=12., nQ=60, Emin=-50., Emax=50., nE=50),
+             position=(0, 0, 10.0))
+    i.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=50, ny=50), position=(0, 0, 11.0))
+    return i
+
+
+def _verdi_taper():
+    """Reference tapered-guide test geometry (tests/components/optics/
+    acc_tapered_guide_instrument.py:87-124) with its VERDI segment file."""
+    from .instrument import instrument
+    from .components.sources.source_simple import Source_simple
+    from .components.optics.guide_tapering import Guide
+    from .components.monitors.psd_monitor import PSD_monitor
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(here, "tests", "golden", "optics", "VERDI_V01_guide1.1")
+    i = instrument("verdi-taper")
+    i.append(Source_simple('source', radius=0., width=0.03, height=0.03, dist=6.35, xw=0.35, yh=0.35,
+                           Lambda0=10., dLambda=9.5, E0=0, dE=0), position=(0, 0, 0.))
+    i.append(Guide('guide', option="file={}".format(path), l=10.99, mx=6., my=6.), position=(0, 0, 6.35))
+    i.append(PSD_monitor('Ixy', nx=250, ny=250, xwidth=0.25, yheight=0.25), position=(0, 0, 6.35 + 10.99 + 1e-3))
+    return i
+
+
+def instruments():
+    from . import configs
+    return {
+        "c1": configs.c1, "c2": configs.c2, "c2s": configs.c2s,
+        "c2-straight": _c2_no_taper, "source-psd": _source_only,
+        "hollow-cylinder-hss": _hollow_cylinder_hss, "verdi-taper": _verdi_taper,
+    }
+
+
+def build_all(verbose=False):
+    from .fuser import fuse
+    out = {}
+    for name, make in instruments().items():
+        out[name] = fuse(make(), verbose=verbose)
+    return out

@@ -1,0 +1,189 @@
+/*
+ * mcv_acc.h -- C ABI of the MI355X neutron ray-tracing engine (libmcvacc.so).
+ *
+ * The reference (mcvine.acc) has no native boundary: its Python component
+ * classes hand `propagate_params` tuples to Numba, which JIT-compiles one fused
+ * CUDA kernel per instrument (acc/run_script.py:175-277) and launches it from
+ * SourceBase.call_process_no_buffer (acc/components/sources/SourceBase.py:36-52).
+ * This header is the boundary that replaces that launch: the Python mirror of
+ * the component API packs each component into an mcv_comp_desc (same parameter
+ * order as the reference's propagate_params), the fuser compiles the chain into
+ * one gfx950 code object, and these entry points own device memory, RNG
+ * streams, launches and histogram read-back.
+ *
+ * Conventions: 0 = success, negative = error (message via mcv_last_error(),
+ * thread-local).  All host arrays are caller-owned, C-contiguous float64/int64.
+ * Device buffers are owned by the mcv_instr handle.  A handle is not
+ * thread-safe; calls block until the device work has finished.
+ */
+#ifndef MCV_ACC_H
+#define MCV_ACC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCV_ABI_VERSION 1
+
+/* ---- error classes (mirrors the exception types the reference raises) ---- */
+#define MCV_OK 0
+#define MCV_ERR_ARG -1     /* bad argument / parameter (RuntimeError, AssertionError) */
+#define MCV_ERR_HIP -2     /* HIP runtime failure (no device, launch failure) */
+#define MCV_ERR_MODULE -3  /* code object missing or kernel symbol absent */
+#define MCV_ERR_COUNT -4   /* processed-neutron count mismatch (run_script.py:218-222) */
+#define MCV_ERR_NOMEM -5
+
+/* ---- component type tags ------------------------------------------------
+ * params/iparams/data layouts (float64 unless stated):
+ *
+ * MCV_ARM                no parameters (acc/components/optics/arm.py:14-43)
+ * MCV_SOURCE_SIMPLE      params[13] = square, width, height, radius, wl_distr,
+ *                        Lambda0, dLambda, E0, dE, xw, yh, dist, pmul
+ *                        (acc/components/sources/source_simple.py:211-215)
+ * MCV_GUIDE              params[10] = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W
+ *                        (acc/components/optics/guide.py:45-50)
+ * MCV_GUIDE_TAPERING     params[9] = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W
+ *                        iparams[1] = nseg; data[10*nseg] = rows w1, w2, h1, h2,
+ *                        ww, hh, whalf, hhalf, lwhalf, lhhalf (row-major, 10 x nseg)
+ *                        (acc/components/optics/guide_tapering.py:57-75,117-139)
+ * MCV_SAMPLE_HSS         single scatterer (homogeneous_single_scatterer.py:26-102)
+ * MCV_SAMPLE_HMS         multiple scatterer (homogeneous_multiple_scatterer.py:22-342)
+ *     iparams[8] = n_shape_nodes, kernel_type, max_scattered_neutrons,
+ *                  max_ms_loops, max_ms_loops_path1, kernel_data_offset,
+ *                  kernel_n0, kernel_n1
+ *     params[16] = mu (absorption coefficient at 2200 m/s, 1/m),
+ *                  sigma (scattering coefficient, 1/m),
+ *                  minimum_neutron_event_probability, kernel params[13]
+ *     data = CSG nodes (MCV_CSG_NODE_SIZE doubles each, root = node 0),
+ *            then kernel tables starting at kernel_data_offset.
+ * MCV_MON_PSD            params[4] = xmin, xmax, ymin, ymax; iparams[2] = nx, ny
+ *                        (acc/components/monitors/psd_monitor.py:66-69)
+ * MCV_MON_WAVELENGTH     params[6] = xmin, xmax, ymin, ymax, Lmin, Lmax;
+ *                        iparams[1] = nchan (wavelength_monitor.py:136-139)
+ * MCV_MON_IQE            params[9] = Ei, Qmin, Qmax, Emin, Emax, max_angle_in_plane,
+ *                        min_angle_in_plane, max_angle_out_of_plane,
+ *                        min_angle_out_of_plane; iparams[2] = nQ, nE
+ *                        (acc/components/monitors/iqe_monitor.py:214-220)
+ * MCV_DET_TUBES          cylindrical array of vertical tubes (no reference
+ *                        counterpart): params[8] = radius, height, phi_min,
+ *                        phi_max (deg), tof_min, tof_max (s), tube_diameter,
+ *                        unused; iparams[3] = ntubes, npixels, ntof
+ *
+ * Monitors own a (3, n_bins) float64 histogram: rows N, sum p, sum p^2.
+ */
+#define MCV_ARM 0
+#define MCV_SOURCE_SIMPLE 1
+#define MCV_GUIDE 2
+#define MCV_GUIDE_TAPERING 3
+#define MCV_SAMPLE_HSS 4
+#define MCV_SAMPLE_HMS 5
+#define MCV_MON_PSD 10
+#define MCV_MON_WAVELENGTH 11
+#define MCV_MON_IQE 12
+#define MCV_DET_TUBES 13
+
+/* scattering kernels of samples (acc/kernels/__init__.py:8-22,363-371) */
+#define MCV_KERNEL_ISOTROPIC 1  /* acc/kernels/isotropic.py:5-21 */
+#define MCV_KERNEL_SQE_GRID 2   /* S(Q,E) on a regular grid (no reference) */
+#define MCV_KERNEL_PHONON_DOS 3 /* 3-peak phonon DOS kernel (no reference) */
+
+/* CSG nodes: {type, a, b, c, left, right} (acc/geometry/arrow_intersect.py:18-123) */
+#define MCV_CSG_NODE_SIZE 6
+#define MCV_CSG_SPHERE 1       /* a = radius */
+#define MCV_CSG_CYLINDER 2     /* a = radius, b = height; axis = local z */
+#define MCV_CSG_BLOCK 3        /* a = width (x), b = height (y), c = thickness (z) */
+#define MCV_CSG_UNION 10
+#define MCV_CSG_DIFFERENCE 11
+#define MCV_CSG_INTERSECTION 12
+
+/* run flags */
+#define MCV_FLAG_GATE_DEAD 1u  /* drop a neutron once p <= 0 (mcvine semantics);
+                                  clear = acc fused-kernel semantics, where an
+                                  absorbed neutron keeps flowing downstream */
+
+typedef struct mcv_comp_desc {
+  int32_t type;
+  int32_t n_params;
+  const double* params;
+  int32_t n_iparams;
+  const int64_t* iparams;
+  int64_t n_data;
+  const double* data;
+  int64_t n_bins; /* monitors: bins of the (3, n_bins) histogram; else 0 */
+} mcv_comp_desc;
+
+typedef struct mcv_run_stats {
+  uint64_t processed;     /* neutron counter (run_script.py:195) */
+  int64_t nblocks;        /* reference partition (SourceBase.py:40-43) */
+  int64_t nthreads;       /* streams */
+  int64_t npt;            /* neutrons per stream */
+  double ms_rng_init;     /* device time of the stream-seeding kernel */
+  double ms_kernel;       /* device time of the fused propagate kernel(s) */
+  double ms_total;        /* host wall time of the call */
+} mcv_run_stats;
+
+typedef struct mcv_instr mcv_instr;
+
+/* Version of this ABI (MCV_ABI_VERSION). */
+int mcv_abi_version(void);
+
+/* Message of the last failing call on this host thread. */
+const char* mcv_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int mcv_device_count(void);
+
+/* Build an instrument handle from a component chain.  Replaces the reference's
+ * InstrumentWrapper (acc/run_script.py:336-346): offsets[(n-1)*3] and
+ * rotmats[(n-1)*9] are the frame changes between consecutive components
+ * (calcTransformations, run_script.py:279-300).  code_object/kernel_name name
+ * the fused kernel that the fuser compiled for exactly this chain. */
+mcv_instr* mcv_instr_create(const mcv_comp_desc* comps, int32_t ncomp,
+                            const double* offsets, const double* rotmats,
+                            const char* code_object, const char* kernel_name);
+
+/* Replace the float parameters of component `comp` (same layout as at create
+ * time) without recompiling: the reference bakes them in at JIT time
+ * (homogeneous_single_scatterer.py:36-40); here they are device data. */
+int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params,
+                         int32_t n_params);
+
+/* Run ncount neutrons.  Replaces the generated run() and
+ * SourceBase.call_process_no_buffer (run_script.py:348-361, SourceBase.py:36-52):
+ * the reference partition (ntotalthreads, threads_per_block -> nblocks, npt)
+ * and numba's xoroshiro128+ seeding are reproduced, so stream tid runs neutrons
+ * [tid*npt, min((tid+1)*npt, ncount)).  Histograms accumulate across calls
+ * until mcv_monitor_clear.  ntotalthreads/tpb <= 0 select 1e6 / 512
+ * (acc/config.py:1-2).  flags: MCV_FLAG_*. */
+int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed,
+                  int64_t ntotalthreads, int32_t tpb, uint32_t flags,
+                  mcv_run_stats* stats);
+
+/* Copy the (3, n_bins) histogram of monitor component `comp` into out3. */
+int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3);
+
+/* Zero every monitor histogram of the handle. */
+int mcv_monitor_clear(mcv_instr* h);
+
+/* Parity support: on the next mcv_instr_run, record the final neutron state
+ * (10 doubles, after the last component) of neutron indices [first, first+count)
+ * into the handle; mcv_instr_read_dump copies them out (count x 10) together
+ * with a per-neutron flag (1 = reached the end of the chain alive). */
+int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count);
+int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive);
+
+/* Seed n xoroshiro128+ streams exactly as numba's create_xoroshiro128p_states
+ * (SplitMix64(seed) into both words, stream i = stream i-1 jumped 2^64) and copy
+ * the (n, 2) uint64 states to host memory.  Used to pin the device seeding. */
+int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out);
+
+/* Release the handle and its device memory. */
+void mcv_instr_destroy(mcv_instr* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCV_ACC_H */
