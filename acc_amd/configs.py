@@ -101,4 +101,80 @@ def c2s():
     return instr
 
 
-BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s}
+def sqe_ridge_grid(nQ=200, nE=200, Qmin=0.0, Qmax=10.0, Emin=-40.0, Emax=40.0, seed=0):
+    """S(Q,E) node values (Q_i = Qmin + i dQ, dQ = (Qmax-Qmin)/nQ; same for E):
+    a Gaussian ridge E(Q) = 30 |sin(Q/2)| meV with sigma 2 meV, plus a flat 5%
+    background, with 2% multiplicative noise from default_rng(seed)."""
+    Q = Qmin + (Qmax - Qmin) / nQ * np.arange(nQ)
+    E = Emin + (Emax - Emin) / nE * np.arange(nE)
+    QQ, EE = np.meshgrid(Q, E, indexing="ij")
+    ridge = np.exp(-(EE - 30.0 * np.abs(np.sin(QQ / 2))) ** 2 / (2 * 2.0 ** 2))
+    S = ridge + 0.05 * ridge.max()
+    noise = 1 + 0.02 * np.random.default_rng(seed).standard_normal(S.shape)
+    return np.clip(S * noise, 0, None), (Qmin, Qmax, Emin, Emax)
+
+
+def sqe_grid_kernel(**kw):
+    from .kernels import SQEGridKernel
+    S, (Qmin, Qmax, Emin, Emax) = sqe_ridge_grid(**kw)
+    mu = AL_NUMBER_DENSITY * 2.0 * BARN
+    sigma = AL_NUMBER_DENSITY * 5.0 * BARN
+    return SQEGridKernel(S, Qmin, Qmax, Emin, Emax, absorption_coefficient=mu, scattering_coefficient=sigma)
+
+
+def phonon_dos_kernel(seed=0):
+    """Three Gaussian DOS peaks at 10/25/40 meV, widths U(1, 4) meV, amplitudes
+    U(0.5, 1.5), both from default_rng(seed)."""
+    from .kernels import PhononDOSKernel
+    rng = np.random.default_rng(seed)
+    widths = rng.uniform(1.0, 4.0, 3)
+    amps = rng.uniform(0.5, 1.5, 3)
+    mu = AL_NUMBER_DENSITY * 2.0 * BARN
+    sigma = AL_NUMBER_DENSITY * 5.0 * BARN
+    return PhononDOSKernel(amps, [10.0, 25.0, 40.0], widths, 0.0, 12.0, -50.0, 50.0, mu, sigma)
+
+
+def tube_array(name="detector", ntubes=400, npixels=128, ntof=500, radius=3.0, height=1.28,
+               phi_min=5.0, phi_max=135.0, tof_min=2.5e-3, tof_max=6.0e-3):
+    from .components.monitors.tube_detector import TubeDetectorArray
+    return TubeDetectorArray(name, radius=radius, height=height, phi_min=phi_min, phi_max=phi_max,
+                             tof_min=tof_min, tof_max=tof_max, ntubes=ntubes, npixels=npixels, ntof=ntof)
+
+
+def hollow_cylinder():
+    """r_in 1.0 cm, r_out 1.5 cm, h 5 cm; the inner cylinder is taller so the
+    hole goes through (axis = local z; the sample is turned to make it vertical)."""
+    from .geometry import Cylinder, Difference
+    return Difference(Cylinder(0.015, 0.05), Cylinder(0.01, 0.06))
+
+
+def c3_single(ntubes=400, npixels=128, ntof=500):
+    """c1 source -> hollow cylinder with the S(Q,E)-grid kernel, single
+    scattering -> tube array."""
+    from .components.samples.homogeneous_single_scatterer import factory
+    instr = new_instrument("c3-single")
+    instr.append(_source(60.0, 10.0), position=(0, 0, 0))
+    HSS = factory(shape=hollow_cylinder(), kernel=sqe_grid_kernel())
+    instr.append(HSS('sample'), position=(0, 0, 10.0), orientation=(90, 0, 0))
+    instr.append(tube_array(ntubes=ntubes, npixels=npixels, ntof=ntof), position=(0, 0, 10.0))
+    return instr
+
+
+def c5_single():
+    """c1 source -> sphere with the phonon-DOS kernel -> I(Q,E) monitor."""
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .geometry import Sphere
+    instr = new_instrument("c5-single")
+    instr.append(_source(60.0, 10.0), position=(0, 0, 0))
+    HSS = factory(shape=Sphere(0.01), kernel=phonon_dos_kernel())
+    s = HSS('sample')
+    instr.append(s, position=(0, 0, 10.0))
+    instr.append(IQE_monitor('iqe', Ei=60., Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100,
+                             min_angle_in_plane=-180, max_angle_in_plane=180,
+                             min_angle_out_of_plane=-90, max_angle_out_of_plane=90),
+                 position=(0, 0, 0), relativeTo=s)
+    return instr
+
+
+BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3s": c3_single, "c5s": c5_single}

@@ -4,7 +4,8 @@
 // (cited per function).  Parameters arrive as a pointer into the packed
 // parameter blob of the instrument (layouts in include/mcv_acc.h); all lanes
 // read the same addresses, so they compile to scalar loads.  Arithmetic keeps
-// the reference's operation order; FMA contraction is left to the compiler.
+// the reference's operation order and the kernels are compiled without FMA
+// contraction, so each operation rounds as the reference's does.
 #pragma once
 #include "mcv_core.h"
 

@@ -74,6 +74,8 @@ def instruments():
         "c1": configs.c1, "c2": configs.c2, "c2s": configs.c2s,
         "c2-straight": _c2_no_taper, "source-psd": _source_only,
         "hollow-cylinder-hss": _hollow_cylinder_hss, "verdi-taper": _verdi_taper,
+        "c3s": configs.c3_single, "c3s-small": lambda: configs.c3_single(ntubes=40, npixels=16, ntof=50),
+        "c5s": configs.c5_single,
     }
 
 

@@ -1,0 +1,158 @@
+#!/usr/bin/env python
+"""Benchmark: neutrons/s of the fused propagate path on one MI355X (FP64).
+
+A step is one run of the workload instrument over `ncount` synthetic neutrons
+(reference partition, numba-seeded xoroshiro128+ streams), with the
+instrument's parameters and tables already resident in HBM: device-side stream
+seeding + the fused kernel + the processed-count check.  Default workload:
+the guide-transport chain (Source_simple -> 10 m guide -> 100-segment elliptic
+tapered guide -> wavelength + PSD monitors), 1e8 neutrons per step.
+
+Prints one JSON line.  `roofline` prices the fused kernel against the FP64
+vector peak with the algorithmic work counted by the oracle's counting build
+(mean FP64 ops per neutron, transcendentals weighted 20); `cpu_baseline` times
+the CPU oracle on this host's cores over a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "neutrons/s per instrument at 1 MI355X (FP64); adjoint/forward cost ratio"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": ("guide transport: Source_simple (r 5 cm, E 5-100 meV) -> 10 m straight guide 5x5 cm m=2 -> "
+           "10 m 100-segment elliptic guide 6->3 cm -> wavelength (200 bins) + PSD 100x100", int(1e8)),
+    "c2s": ("guide transport + isotropic sphere sample (r 1 cm) -> I(Q,E) 120x100 + PSD 250x250", int(1e8)),
+    "c1": ("Source_simple (E 60+-10 meV) -> isotropic sphere (r 1 cm, 5 b / 2 b) -> I(Q,E) 120x100 + "
+           "PSD 250x250", int(1e7)),
+}
+
+
+def oracle_flops_per_neutron(instr, n=1000000, seed=0):
+    """Mean FP64 ops per neutron over the first n neutrons (counting oracle)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    from acc_amd.instrument import calcTransformations
+    path = os.path.join(ROOT, "oracle", "liboracle_flops.so")
+    L = ctypes.CDLL(path)
+    saved_lib, saved_path = pyoracle._lib, pyoracle.LIB
+    try:
+        pyoracle._lib, pyoracle.LIB = None, path
+        pyoracle.lib()
+        L.orc_flops_read.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_flops_reset()
+        descs = [c.mcv_desc() for c in instr.components]
+        off, rot = calcTransformations(instr)
+        _, processed, _, _ = pyoracle.run(descs, off, rot, n, seed, gate=True, nthreads=1)
+        c = np.zeros(3, dtype=np.uint64)
+        L.orc_flops_read(c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    finally:
+        pyoracle._lib, pyoracle.LIB = saved_lib, saved_path
+    return {"arith": float(c[0]) / processed, "trans": float(c[1]) / processed,
+            "draws": float(c[2]) / processed, "F_alg": (float(c[0]) + 20.0 * float(c[1])) / processed,
+            "neutrons": int(processed)}
+
+
+def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0):
+    """Oracle on this host's cores over a bounded leading part of the workload
+    (whole streams of the reference partition)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from acc_amd.instrument import calcTransformations
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    descs = [c.mcv_desc() for c in instr.components]
+    off, rot = calcTransformations(instr)
+    nblocks, nthreads, npt = pyoracle.partition(ncount, ntotalthreads, tpb)
+    # probe, then size the sample to ~`seconds`
+    probe = max(1, min(nthreads, 2000000 // max(npt, 1)))
+    t = time.perf_counter()
+    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, True, (0, probe), cores)
+    dt = time.perf_counter() - t
+    rate = done / dt
+    streams = min(nthreads, max(probe, int(rate * seconds / max(npt, 1))))
+    t = time.perf_counter()
+    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, True, (0, streams), cores)
+    dt = time.perf_counter() - t
+    return {"value": done / dt, "unit": "neutrons/s", "cores": cores, "kind": "port",
+            "sample": "first %d of %d streams of the same partition (%d neutrons, seed 0), %.1f s" % (
+                streams, nthreads, done, dt)}
+
+
+def traffic_from_profiles(workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--ncount", type=float, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--falg-neutrons", type=int, default=1000000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
+              "(multi-GPU sharding by stream ranges is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        return 2
+
+    from acc_amd import configs
+    from acc_amd._abi import MCV_FLAG_GATE_DEAD
+    from acc_amd.run_script import get_handle
+
+    desc, default_n = WORKLOADS[a.workload]
+    ncount = int(a.ncount or default_n)
+    instr = configs.BUILDERS[a.workload]()
+    h = get_handle(instr)
+    seed = 0
+    for _ in range(a.warmup):
+        h.clear()
+        h.run(ncount, seed, 0, 0, MCV_FLAG_GATE_DEAD)
+    kernel_ms, rng_ms = [], []
+    h.clear()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        st = h.run(ncount, seed + k, 0, 0, MCV_FLAG_GATE_DEAD)
+        kernel_ms.append(st["ms_kernel"])
+        rng_ms.append(st["ms_rng_init"])
+    elapsed = time.perf_counter() - t0
+    value = ncount * a.steps / elapsed
+
+    fl = oracle_flops_per_neutron(configs.BUILDERS[a.workload](), n=a.falg_neutrons)
+    k_avg = sum(kernel_ms) / len(kernel_ms)
+    achieved = fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles(a.workload),
+                "kernel": "mcv_fused (%s)" % a.workload, "kernel_ms_avg": k_avg,
+                "falg_ops_per_neutron": fl["F_alg"], "falg_detail": fl,
+                "rng_seed_kernel_ms_avg": sum(rng_ms) / len(rng_ms)}
+    cpu = None if a.no_cpu_baseline else cpu_baseline(configs.BUILDERS[a.workload](), ncount)
+    line = {
+        "metric": METRIC, "value": value, "unit": "neutrons/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: {desc}", "ncount_per_step": ncount,
+                   "partition": "reference default (ntotalthreads 1e6, 512 threads/block)",
+                   "semantics": "dead neutrons (p<=0) dropped"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

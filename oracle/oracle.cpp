@@ -34,6 +34,13 @@
 
 #include "../include/mcv_acc.h"
 
+// Flop-counting hooks (oracle_flops.cpp defines them; no-ops here).
+#ifndef ORC_PAUSE
+#define ORC_PAUSE ((void)0)
+#define ORC_RESUME ((void)0)
+#define ORC_DRAW ((void)0)
+#endif
+
 namespace orc {
 
 // mcni.utils.conversion (McStas constants, not vendored)
@@ -85,7 +92,10 @@ static inline void jump(Rng& s) {
 
 static inline float uniform_f32(Rng& s) {
   uint64_t x = next(s);
+  ORC_DRAW;
+  ORC_PAUSE;  // the draw's integer-to-float conversion is RNG work, counted apart
   double d = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  ORC_RESUME;
   return (float)d;
 }
 
@@ -172,6 +182,7 @@ static double randvec_target_rect(const double* target, double width, double hei
                                   double* vecout) {
   double dx = width * ((double)rand1 * 2 - 1) / 2.0;
   double dy = height * ((double)rand2 * 2 - 1) / 2.0;
+  ORC_PAUSE;  // loop-invariant: depends on the target only
   double dist = length(target);
   double p1[3], vertical[3] = {0.0, 1.0, 0.0};
   cross(target, vertical, p1);
@@ -179,6 +190,7 @@ static double randvec_target_rect(const double* target, double width, double hei
   double p2[3];
   cross(target, p1, p2);
   normalize(p2);
+  ORC_RESUME;
   scale(p1, dx);
   scale(p2, dy);
   double tmp[3];
@@ -1341,19 +1353,19 @@ struct Comp {
 
 static Sample make_sample(const mcv_comp_desc& d) {
   Sample S;
-  S.shape.nodes = d.data;
+  S.shape.nodes = (const double*)d.data;
   S.shape.n = (int)d.iparams[0];
   S.kernel = (int)d.iparams[1];
   S.max_scattered = (int)d.iparams[2];
   S.max_ms_loops = (int)d.iparams[3];
   S.max_ms_loops_path1 = (int)d.iparams[4];
-  S.kdata = d.data + d.iparams[5];
+  S.kdata = (const double*)d.data + d.iparams[5];
   S.kn0 = d.iparams[6];
   S.kn1 = d.iparams[7];
   S.mu = d.params[0];
   S.sigma = d.params[1];
   S.min_prob = d.params[2];
-  S.kp = d.params + 3;
+  S.kp = (const double*)d.params + 3;
   return S;
 }
 
@@ -1451,9 +1463,9 @@ static bool build_chain(const mcv_comp_desc* comps, int ncomp, const double* off
   for (int i = 0; i < ncomp; ++i) {
     Comp c;
     c.type = comps[i].type;
-    c.P = comps[i].params;
+    c.P = (const double*)comps[i].params;
     c.IP = comps[i].iparams;
-    c.data = comps[i].data;
+    c.data = (const double*)comps[i].data;
     c.nbins = comps[i].n_bins;
     c.hist_off = C.hist_total;
     C.hist_total += 3 * c.nbins;
@@ -1544,9 +1556,9 @@ int orc_component(const mcv_comp_desc* d, double* n, uint64_t* state2, double* h
   Rng rng{state2[0], state2[1]};
   int ret = 0;
   switch (d->type) {
-    case MCV_SOURCE_SIMPLE: source_simple(rng, n, d->params); break;
-    case MCV_GUIDE: guide(n, d->params); break;
-    case MCV_GUIDE_TAPERING: guide_tapering(n, d->params, d->iparams[0], d->data); break;
+    case MCV_SOURCE_SIMPLE: source_simple(rng, n, (const double*)d->params); break;
+    case MCV_GUIDE: guide(n, (const double*)d->params); break;
+    case MCV_GUIDE_TAPERING: guide_tapering(n, (const double*)d->params, d->iparams[0], (const double*)d->data); break;
     case MCV_SAMPLE_HSS: {
       Sample S = make_sample(*d);
       hss_propagate(rng, n, S);
@@ -1562,10 +1574,10 @@ int orc_component(const mcv_comp_desc* d, double* n, uint64_t* state2, double* h
         ret = interactM1(rng, S, out, n);
       break;
     }
-    case MCV_MON_PSD: psd_monitor(n, d->params, d->iparams[0], d->iparams[1], hist); break;
-    case MCV_MON_WAVELENGTH: wavelength_monitor(n, d->params, d->iparams[0], hist); break;
-    case MCV_MON_IQE: iqe_monitor(n, d->params, d->iparams[0], d->iparams[1], hist); break;
-    case MCV_DET_TUBES: tube_detector(n, d->params, d->iparams[0], d->iparams[1], d->iparams[2], hist); break;
+    case MCV_MON_PSD: psd_monitor(n, (const double*)d->params, d->iparams[0], d->iparams[1], hist); break;
+    case MCV_MON_WAVELENGTH: wavelength_monitor(n, (const double*)d->params, d->iparams[0], hist); break;
+    case MCV_MON_IQE: iqe_monitor(n, (const double*)d->params, d->iparams[0], d->iparams[1], hist); break;
+    case MCV_DET_TUBES: tube_detector(n, (const double*)d->params, d->iparams[0], d->iparams[1], d->iparams[2], hist); break;
     default: break;
   }
   state2[0] = rng.s0;

@@ -38,6 +38,12 @@ def compare_states(g, o, ga, oa):
     per[~both] = 0
     flips = (~same_alive) | (per > 1e-6)
     good = both & ~flips
+    if good.any():
+        fields = rel[good].max(axis=0)
+        worst = np.flatnonzero(good)[np.argmax(per[good])]
+        print("[parity] per-field max rel err:", " ".join("%.1e" % v for v in fields))
+        print("[parity] worst neutron %d gpu %s" % (worst, np.array2string(g[worst], precision=17)))
+        print("[parity] worst neutron %d orc %s" % (worst, np.array2string(o[worst], precision=17)))
     return (per[good].max() if good.any() else 0.0), int(flips.sum()), int(good.sum())
 
 
@@ -65,6 +71,9 @@ CASES = [
     ("c2s", True, 200000, 4096),
     ("hollow-cylinder-hss", True, 300000, 8192),
     ("verdi-taper", True, 60000, 2048),
+    ("c3s-small", True, 300000, 8192),
+    ("c5s", True, 300000, 8192),
+    ("c5s", False, 100000, 4096),
 ]
 
 
