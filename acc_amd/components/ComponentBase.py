@@ -19,8 +19,8 @@ class DeviceFunction:
     """Marker for a component's device routine.
 
     `call` is a format string producing the C++ statement(s) for component
-    index {i}; the fused kernel exposes `rng`, `n` (the neutron), `P` (packed
-    parameters), `H` (histograms) and the launch block `A`.
+    index {i} acting on the neutron variable {n}; the fused kernel exposes
+    `rng`, `P` (packed parameters), `H` (histograms) and the launch block `A`.
     """
 
     def __init__(self, call, stochastic=False, monitor=False, preamble=None):
@@ -29,8 +29,8 @@ class DeviceFunction:
         self.monitor = monitor
         self.preamble = preamble
 
-    def render(self, comp, i):
-        return self.call.format(i=i, **comp._render_context(i))
+    def render(self, comp, i, n="n"):
+        return self.call.format(i=i, n=n, **comp._render_context(i))
 
     def render_preamble(self, comp, i):
         if self.preamble is None:

@@ -44,7 +44,7 @@ class Guide(ComponentBase):
             float(R0), float(Qc), float(alpha), float(m), float(W),
         )
 
-    propagate = DeviceFunction("mcv::guide(n, P + A.off[{i}]);")
+    propagate = DeviceFunction("mcv::guide({n}, P + A.off[{i}]);")
 
     def _mcv_desc(self):
         return dict(type=self.mcv_type, params=np.array(self.propagate_params, dtype=float))

@@ -81,7 +81,7 @@ class Guide(ComponentBase):
         w2 = np.asarray(w2[:-1], dtype=np.float64)
         return h1, h2, w1, w2
 
-    propagate = DeviceFunction("mcv::guide_tapering(n, P + A.off[{i}], P + A.doff[{i}]);")
+    propagate = DeviceFunction("mcv::guide_tapering({n}, P + A.off[{i}], P + A.doff[{i}]);")
 
     def _mcv_desc(self):
         nseg, scalars, data = self.propagate_params

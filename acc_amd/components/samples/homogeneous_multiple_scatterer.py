@@ -39,7 +39,7 @@ def factory(
             self.propagate_params = ()
 
         propagate = DeviceFunction(
-            "mcv::hms_scatterM<{shape}, {kind}, {M}>(rng, n, P + A.off[{i}], P + A.doff[{i}], out)",
+            "mcv::hms_scatterM<{shape}, {kind}, {M}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}], out)",
             stochastic=True, preamble=SampleBase._shape_preamble)
 
         def _render_context(self, i):

@@ -26,7 +26,7 @@ def factory(shape, kernel):
             self.propagate_params = ()
 
         propagate = DeviceFunction(
-            "mcv::hss_propagate<{shape}, {kind}>(rng, n, P + A.off[{i}], P + A.doff[{i}]);",
+            "mcv::hss_propagate<{shape}, {kind}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}]);",
             stochastic=True, preamble=SampleBase._shape_preamble)
 
     HomogeneousSingleScattererBase.shape = shape

@@ -55,7 +55,7 @@ class Source_simple(SourceBase):
             xw, yh, dist, pmul
         )
 
-    propagate = DeviceFunction("mcv::source_simple(rng, n, P + A.off[{i}]);", stochastic=True)
+    propagate = DeviceFunction("mcv::source_simple(rng, {n}, P + A.off[{i}]);", stochastic=True)
 
     def _mcv_desc(self):
         return dict(type=self.mcv_type, params=np.array([float(v) for v in self.propagate_params]))

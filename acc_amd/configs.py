@@ -177,4 +177,20 @@ def c5_single():
     return instr
 
 
-BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3s": c3_single, "c5s": c5_single}
+def c3(ntubes=400, npixels=128, ntof=500, max_ms_loops=2, max_ms_loops_path1=3, max_scattered_neutrons=10):
+    """c1 source -> hollow cylinder with the S(Q,E)-grid kernel and multiple
+    scattering -> 400 x 128 x 500 tube array.  "Up to 3 scattering orders"
+    maps onto the reference scatterer's loop bounds as 3 interactions per
+    encounter with the sample (max_ms_loops_path1) and 2 encounters
+    (max_ms_loops: the two walls of the hollow cylinder)."""
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    instr = new_instrument("c3")
+    instr.append(_source(60.0, 10.0), position=(0, 0, 0))
+    HMS = factory(shape=hollow_cylinder(), kernel=sqe_grid_kernel(), max_scattered_neutrons=max_scattered_neutrons,
+                  max_ms_loops=max_ms_loops, max_ms_loops_path1=max_ms_loops_path1)
+    instr.append(HMS('sample'), position=(0, 0, 10.0), orientation=(90, 0, 0))
+    instr.append(tube_array(ntubes=ntubes, npixels=npixels, ntof=ntof), position=(0, 0, 10.0))
+    return instr
+
+
+BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3": c3, "c3s": c3_single, "c5s": c5_single}

@@ -1,0 +1,326 @@
+// mcv_hms.h -- propagation helpers and the homogeneous multiple scatterer on
+// the device.
+//
+// acc/geometry/propagation.py:9-191 and
+// acc/components/samples/homogeneous_multiple_scatterer.py:63-303, restated
+// over compile-time bounded root lists (Shape::K).  The breadth-first neutron
+// buffers of the reference (to_be_scattered, to_be_scattered2, scattered and
+// the output list, M x 10 doubles each) are per-lane arrays indexed at run
+// time, so they live in the lane's private scratch.
+#pragma once
+#include "mcv_samples.h"
+
+namespace mcv {
+
+// forward_intersect: roots >= 0 shifted to the front (propagation.py:11-22)
+template <class Shape>
+__device__ __forceinline__ int forward_intersect(const double* S, double x, double y, double z, double vx, double vy,
+                                                 double vz, double* out) {
+  constexpr int K = Shape::K;
+  double ts[K];
+  const int N = Shape::intersect(S, x, y, z, vx, vy, vz, ts);
+  int Nm = N;
+#pragma unroll
+  for (int i = K - 1; i >= 0; --i)
+    if (i < N && ts[i] >= 0) Nm = i;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    double v = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (j == i + Nm) v = ts[j];
+    out[i] = v;
+  }
+  return N - Nm;
+}
+
+template <class Shape>
+__device__ __forceinline__ int locate0(const double* S, double x, double y, double z) {
+  return Shape::locate(S, x, y, z);
+}
+
+// propagation.py:24-50
+template <class Shape>
+__device__ __forceinline__ bool is_exiting(const double* S, const Neutron& n) {
+  constexpr int K = Shape::K;
+  const int loc = locate0<Shape>(S, n.x, n.y, n.z);
+  if (loc == INSIDE) return false;
+  double ts[K];
+  const int N = forward_intersect<Shape>(S, n.x, n.y, n.z, n.vx, n.vy, n.vz, ts);
+  if (N == 0) return true;
+  if (loc == OUTSIDE) return false;
+  double previous = 0.;
+#pragma unroll
+  for (int it = 0; it < K; ++it) {
+    if (it < N) {
+      const double now = ts[it];
+      const double middle = (previous + now) / 2.;
+      if (locate0<Shape>(S, n.x + n.vx * middle, n.y + n.vy * middle, n.z + n.vz * middle) != ONBORDER) return false;
+      previous = now;
+    }
+  }
+  return true;
+}
+
+// propagation.py:52-59
+template <class Shape>
+__device__ __forceinline__ void propagate_out(const double* S, Neutron& n) {
+  constexpr int K = Shape::K;
+  double ts[K];
+  const int N = forward_intersect<Shape>(S, n.x, n.y, n.z, n.vx, n.vy, n.vz, ts);
+  if (N == 0) return;
+  double last = ts[0];
+#pragma unroll
+  for (int i = 1; i < K; ++i)
+    if (i == N - 1) last = ts[i];
+  prop_dt_inplace(n, last);
+}
+
+// propagation.py:61-105
+template <class Shape>
+__device__ __forceinline__ void propagate_to_next_incident_surface(const double* S, Neutron& n) {
+  constexpr int K = Shape::K;
+  const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+  const int loc = locate0<Shape>(S, x, y, z);
+  if (loc == INSIDE) return;
+  double ts[K];
+  const int N = forward_intersect<Shape>(S, x, y, z, vx, vy, vz, ts);
+  if (N == 0) return;
+  double t = 0;
+  if (loc == OUTSIDE) {
+    t = ts[0];
+  } else {
+    bool found = false, done = false;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if (i < N && !done) {
+        const int loc1 = locate0<Shape>(S, x + ts[i] / 2 * vx, y + ts[i] / 2 * vy, z + ts[i] / 2 * vz);
+        if (loc1 == INSIDE) return;
+        if (loc1 == OUTSIDE) {
+          t = ts[i];
+          found = true;
+          done = true;
+        }
+      }
+    }
+    if (!found) return;
+  }
+  prop_dt_inplace(n, t);
+}
+
+// propagation.py:107-152
+template <class Shape>
+__device__ __forceinline__ void propagate_to_next_exiting_surface(const double* S, Neutron& n) {
+  constexpr int K = Shape::K;
+  double ts[K];
+  const int loc = locate0<Shape>(S, n.x, n.y, n.z);
+  double t;
+  if (loc == INSIDE) {
+    forward_intersect<Shape>(S, n.x, n.y, n.z, n.vx, n.vy, n.vz, ts);
+    t = ts[0];
+  } else {
+    if (loc == OUTSIDE) propagate_to_next_incident_surface<Shape>(S, n);
+    const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+    const int N = forward_intersect<Shape>(S, x, y, z, vx, vy, vz, ts);
+    bool found = false, done = false;
+    t = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if (i < N && !done) {
+        const int loc1 = locate0<Shape>(S, x + ts[i] / 2 * vx, y + ts[i] / 2 * vy, z + ts[i] / 2 * vz);
+        if (loc1 == OUTSIDE) return;
+        if (loc1 == INSIDE) {
+          t = ts[i];
+          found = true;
+          done = true;
+        }
+      }
+    }
+    if (!found) return;
+  }
+  prop_dt_inplace(n, t);
+}
+
+// propagation.py:154-191
+template <class Shape>
+__device__ __forceinline__ double tof_before_first_exit(const double* S, const Neutron& n) {
+  constexpr int K = Shape::K;
+  const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+  const int loc = locate0<Shape>(S, x, y, z);
+  if (loc == OUTSIDE) return 0.;
+  double ts[K];
+  const int N = forward_intersect<Shape>(S, x, y, z, vx, vy, vz, ts);
+  if (N == 0) return 0.;
+  if (loc == INSIDE) return ts[0];
+  double t = 0.;
+  bool done = false;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i < N && !done) {
+      const int loc1 = locate0<Shape>(S, x + ts[i] / 2 * vx, y + ts[i] / 2 * vy, z + ts[i] / 2 * vz);
+      if (loc1 == OUTSIDE) return 0.;
+      if (loc1 == INSIDE) {
+        t = ts[i];
+        done = true;
+      }
+    }
+  }
+  return t;
+}
+
+__device__ __forceinline__ bool is_moving(const Neutron& n) {  // acc/neutron.py:60-65
+  return sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz) > 1.e-6;
+}
+
+__device__ __forceinline__ double atten_coeff(const double* P, const Neutron& n) {  // :63-67
+  return P[1] + absorption_coeff(P, n);
+}
+
+// :69-79
+template <class Shape>
+__device__ __forceinline__ void propagate_to_next_exiting_surface_with_attenuation(const double* P, const double* D,
+                                                                                   Neutron& n) {
+  const double vx = n.vx, vy = n.vy, vz = n.vz;
+  const double tof = tof_before_first_exit<Shape>(D, n);
+  const double v = sqrt(vx * vx + vy * vy + vz * vz);
+  const double distance = tof * v;
+  const double atten = atten_coeff(P, n);
+  const double transmission_prob = exp(-atten * distance);
+  propagate_to_next_exiting_surface<Shape>(D, n);
+  n.p *= transmission_prob;
+}
+
+// :81-121; out[0] transmitted, out[1] scattered
+template <class Shape, int KIND>
+__device__ __forceinline__ int interactM1(Rng& rng, const double* P, const double* D, Neutron* out, Neutron& n) {
+  const double vx = n.vx, vy = n.vy, vz = n.vz;
+  const int loc = locate0<Shape>(D, n.x, n.y, n.z);
+  if (loc != INSIDE) propagate_to_next_incident_surface<Shape>(D, n);
+  const double tof = tof_before_first_exit<Shape>(D, n);
+  if (tof == 0.) return 0;
+  const double v = sqrt(vx * vx + vy * vy + vz * vz);
+  const double distance = tof * v;
+  const double sigma = P[1];
+  const double mu = absorption_coeff(P, n);
+  const double transmission_prob = exp(-(mu + sigma) * distance);
+  out[0] = n;
+  propagate_to_next_exiting_surface<Shape>(D, out[0]);
+  out[0].p *= transmission_prob;
+  // the absorption branch writes out[1] and is overwritten below; the kernels'
+  // absorb() is a no-op (acc/kernels/__init__.py:490-492)
+  const double rand = rng.uniform();
+  const double xs = distance * rand;
+  const double prob = distance * exp(-(mu + sigma) * xs);
+  out[1] = n;
+  out[1].p *= prob;
+  prop_dt_inplace(out[1], xs / v);
+  kernel_scatter<KIND>(rng, out[1], P, D);
+  return 2;
+}
+
+// :128-178
+template <class Shape, int KIND, int M>
+__device__ __noinline__ int interactM_path1(Rng& rng, const double* P, const double* D, Neutron* out, int out_len,
+                                            Neutron& n) {
+  const double min_prob = P[2];
+  const int loops = (int)P[20];
+  if (n.p < min_prob) return 0;
+  Neutron bufA[M], bufB[M], scattered[2];
+  Neutron* tbs = bufA;
+  Neutron* tbs2 = bufB;
+  int N_tbs = 1;
+  tbs[0] = n;
+  int out_index = 0;
+  const int lim = M < out_len ? M : out_len;
+  for (int iloop = 0; iloop < loops; ++iloop) {
+    int s2 = 0;
+    for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
+      const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, tbs[ineutron]);
+      for (int is = 0; is < ns1; ++is) {
+        const Neutron& sc1 = scattered[is];
+        if (sc1.p < min_prob) continue;
+        if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) != INSIDE) {
+          out[out_index] = sc1;
+          out_index += 1;
+          if (out_index >= lim) return out_index;
+          continue;
+        }
+        tbs2[s2] = sc1;
+        s2 += 1;
+        if (s2 >= M) break;
+      }
+      if (s2 >= M) break;
+    }
+    Neutron* tmp = tbs;
+    tbs = tbs2;
+    tbs2 = tmp;
+    N_tbs = s2;
+    if (!N_tbs) break;
+  }
+  for (int i = 0; i < N_tbs; ++i) {
+    if (out_index >= lim) break;
+    out[out_index] = tbs[i];
+    out_index += 1;
+  }
+  return out_index;
+}
+
+// :214-273.  P[19] = max_ms_loops, P[20] = max_ms_loops_path1 (packed iparams).
+template <class Shape, int KIND, int M>
+__device__ __noinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
+                                         const double* __restrict__ D, Neutron* out) {
+  if (is_exiting<Shape>(D, n)) {
+    out[0] = n;
+    return 1;
+  }
+  const double min_prob = P[2];
+  const int loops = (int)P[19];
+  Neutron bufA[M], bufB[M], scattered[M];
+  Neutron* tbs = bufA;
+  Neutron* tbs2 = bufB;
+  int N_tbs = 1;
+  tbs[0] = n;
+  int out_index = 0;
+  for (int iloop = 0; iloop < loops; ++iloop) {
+    int s2 = 0;
+    for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
+      const int ns1 = interactM_path1<Shape, KIND, M>(rng, P, D, scattered, M, tbs[ineutron]);
+      for (int is = 0; is < ns1; ++is) {
+        Neutron& sc1 = scattered[is];
+        if (sc1.p < min_prob) continue;
+        if (!is_moving(sc1)) continue;
+        if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) == INSIDE)
+          propagate_to_next_exiting_surface_with_attenuation<Shape>(P, D, sc1);
+        if (is_exiting<Shape>(D, sc1)) {
+          out[out_index] = sc1;
+          out_index += 1;
+          if (out_index >= M) return out_index;
+          continue;
+        }
+        tbs2[s2] = sc1;
+        s2 += 1;
+        if (s2 >= M) break;
+      }
+      if (s2 >= M) break;
+    }
+    Neutron* tmp = tbs;
+    tbs = tbs2;
+    tbs2 = tmp;
+    N_tbs = s2;
+    if (!N_tbs) break;
+  }
+  for (int i = 0; i < N_tbs; ++i) {
+    if (out_index >= M) break;
+    Neutron& nn = tbs[i];
+    const double x = nn.x, y = nn.y, z = nn.z;
+    propagate_out<Shape>(D, nn);
+    const double dist = sqrt((nn.x - x) * (nn.x - x) + (nn.y - y) * (nn.y - y) + (nn.z - z) * (nn.z - z));
+    const double atten = atten_coeff(P, nn);
+    nn.p *= exp(-atten * dist);
+    out[out_index] = nn;
+    out_index += 1;
+  }
+  return out_index;
+}
+
+}  // namespace mcv

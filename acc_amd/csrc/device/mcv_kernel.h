@@ -35,8 +35,9 @@ __device__ __forceinline__ void count_processed(const LaunchArgs& A, int64_t min
 
 }  // namespace mcv
 
-// Per-stream loop around a chain body `bool BODY(Rng&, Neutron&, const LaunchArgs&)`
-// returning whether the neutron reached the end of the chain alive.
+// Per-stream loop around a chain body `int BODY(Rng&, Neutron&, const LaunchArgs&)`
+// returning the dump flag: 1/0 whether the neutron reached the end of the
+// chain alive, or, for multiple-scattering chains, the number of outputs.
 #define MCV_DEFINE_FUSED_KERNEL(NAME, BODY)                                                   \
   extern "C" __global__ void __launch_bounds__(256) NAME(const mcv::LaunchArgs A) {           \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
@@ -51,8 +52,8 @@ __device__ __forceinline__ void count_processed(const LaunchArgs& A, int64_t min
       n.s1 = 0;                                                                               \
       n.s2 = 0;                                                                               \
       for (int64_t idx = start; idx < end; ++idx) {                                           \
-        const bool alive = BODY(rng, n, A);                                                   \
-        mcv::dump_neutron(A, idx, n, alive ? 1 : 0);                                          \
+        const int alive = (int)BODY(rng, n, A);                                               \
+        mcv::dump_neutron(A, idx, n, alive);                                                  \
       }                                                                                       \
       mine = end > start ? end - start : 0;                                                   \
     }                                                                                         \

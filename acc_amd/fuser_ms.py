@@ -1,0 +1,65 @@
+"""Fused kernels for chains with a multiple-scattering sample.
+
+The reference's generated kernel calls the multiple scatterer with a per-thread
+out_neutrons buffer and then runs every downstream component, with its frame
+change, on each output neutron (acc/run_script.py:204-230, template :245-260).
+The same structure is emitted here; the dump records output 0's final state and
+the number of outputs.  As in the reference, one multiple-scattering component
+per chain is supported (run_script.py:211).
+"""
+
+
+def generate_ms_source(instrument):
+    comps = instrument.components
+    ms = [i for i, c in enumerate(comps) if c.is_multiplescattering]
+    if len(ms) != 1:
+        raise NotImplementedError("one multiple-scattering component per instrument is supported")
+    m = ms[0]
+    structure = " -> ".join(type(c).__name__ for c in comps)
+    pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
+    lines = []
+    for i in range(m):
+        c = comps[i]
+        if i > 0:
+            lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
+        lines.append(c.propagate.render(c, i))
+        lines.append("if (A.gate && !(n.p > 0)) return 0;")
+    if m > 0:
+        lines.append(f"mcv::apply_transformation(n, P + A.xoff[{m - 1}]);")
+    M = comps[m].max_scattered_neutrons
+    lines.append(f"mcv::Neutron out[{M}];")
+    lines.append("const int num = " + comps[m].propagate.render(comps[m], m) + ";")
+    tail = []
+    for i in range(m + 1, len(comps)):
+        c = comps[i]
+        tail.append(f"mcv::apply_transformation(o, P + A.xoff[{i - 1}]);")
+        tail.append(c.propagate.render(c, i, n="o"))
+        tail.append("if (A.gate && !(o.p > 0)) break;")
+    tail_text = "\n      ".join(tail)
+    lines.append(f"""for (int k = 0; k < num; ++k) {{
+    mcv::Neutron o = out[k];
+    do {{
+      if (A.gate && !(o.p > 0)) break;
+      {tail_text}
+    }} while (0);
+    if (k == 0) n = o;
+  }}""")
+    lines.append("return num;")
+    body = "\n  ".join(lines)
+    src = f"""// fused instrument kernel (multiple scattering): {structure}
+#include "mcv_kernel.h"
+#include "mcv_hms.h"
+
+namespace {{
+{pre}
+__device__ __forceinline__ int chain_body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)rng; (void)H;
+  {body}
+}}
+}}  // namespace
+
+MCV_DEFINE_FUSED_KERNEL(KERNEL_NAME, chain_body)
+"""
+    return src, structure

@@ -68,6 +68,47 @@ def _verdi_taper():
     return i
 
 
+def _hms_reference():
+    """The reference's multiple-scattering test sample
+    (tests/components/samples/HMS_isotropic_hollowcylinder.py: max_ms_loops=2,
+    max_ms_loops_path1=5, max_scattered_neutrons=100; hollow cylinder r 2/1 cm,
+    h 6/7 cm, coefficients 10/m) behind a small source, with a PSD after it."""
+    from .instrument import instrument
+    from .components.sources.source_simple import Source_simple
+    from .components.samples import loadFirstHomogeneousScatterer
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hs = loadFirstHomogeneousScatterer(os.path.join(here, "tests", "golden", "sampleassemblies",
+                                                    "isotropic_hollowcylinder", "sampleassembly.xml"))
+    HMS = factory(shape=hs.shape(), kernel=hs.kernel(), max_ms_loops=2, max_ms_loops_path1=5,
+                  max_scattered_neutrons=100)
+    i = instrument("hms-ref")
+    i.append(Source_simple('source', radius=0., width=0.01, height=0.01, dist=0.9, xw=0.008, yh=0.008,
+                           Lambda0=10., dLambda=9.5, E0=0, dE=0), position=(0, 0, 0))
+    i.append(HMS('sample'), position=(0, 0, 1.0), orientation=(90, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=50, ny=50), position=(0, 0, 1.5))
+    return i
+
+
+def _csg_chain(xmlname, orientation=(0, 0, 0)):
+    """Single-scatterer sample with a shape from the reference's geometry
+    tests (tests/geometry/*.xml), for the union/intersection/difference code."""
+    from .instrument import instrument
+    from .configs import _source, isotropic_al_kernel
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .geometry import parse_file
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    shape = parse_file(os.path.join(here, "tests", "golden", "geometry", xmlname))[0]
+    i = instrument("csg-" + xmlname)
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    HSS = factory(shape=shape, kernel=isotropic_al_kernel(50.0, 2.0))
+    i.append(HSS('sample'), position=(0, 0, 10.0), orientation=orientation)
+    i.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=50, ny=50), position=(0, 0, 10.5))
+    return i
+
+
 def instruments():
     from . import configs
     return {
@@ -76,6 +117,11 @@ def instruments():
         "hollow-cylinder-hss": _hollow_cylinder_hss, "verdi-taper": _verdi_taper,
         "c3s": configs.c3_single, "c3s-small": lambda: configs.c3_single(ntubes=40, npixels=16, ntof=50),
         "c5s": configs.c5_single,
+        "c3": configs.c3, "c3-small": lambda: configs.c3(ntubes=40, npixels=16, ntof=50),
+        "hms-ref": _hms_reference,
+        "csg-union2": lambda: _csg_chain("union_example2.xml", (90, 0, 0)),
+        "csg-intersection": lambda: _csg_chain("intersection_example1.xml", (90, 0, 0)),
+        "csg-difference": lambda: _csg_chain("difference_example1.xml", (90, 0, 0)),
     }
 
 
