@@ -30,6 +30,7 @@ MCV_MON_PSD = 10
 MCV_MON_WAVELENGTH = 11
 MCV_MON_IQE = 12
 MCV_DET_TUBES = 13
+MCV_SRC_IQE_NORMALIZER = 20
 
 MCV_KERNEL_ISOTROPIC = 1
 MCV_KERNEL_SQE_GRID = 2

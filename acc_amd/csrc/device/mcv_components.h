@@ -341,3 +341,30 @@ __device__ __forceinline__ void tube_detector(const Neutron& n, const double* __
 }
 
 }  // namespace mcv
+
+namespace mcv {
+
+// ------------------------------------------------------------------ IQE normalizer source
+// Neutrons of the I(Q,E) monitor's Monte-Carlo normalizer
+// (acc/components/monitors/iqe_monitor.py:317-344, makeNeutrons): energy
+// transfer E uniform in [Emin, Emax), final speed e2v(Ei - E), direction
+// uniform on 4 pi, weight vf/vi, at the origin.  P = Ei, Emin, Emax.
+__device__ __forceinline__ void iqe_normalizer_source(Rng& rng, Neutron& n, const double* __restrict__ P) {
+  const double Ei = P[0], Emin = P[1], Emax = P[2];
+  const double E = Emin + rng.uniform() * (Emax - Emin);
+  const double Ef = Ei - E;
+  const double vi = sqrt(Ei) * SE2V, vf = sqrt(Ef) * SE2V;
+  const double cos_t = rng.uniform() * 2 - 1;
+  const double sin_t = sin(acos(cos_t));
+  const double phi = rng.uniform() * 2 * PI;
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  n.x = 0; n.y = 0; n.z = 0;
+  n.vx = vf * sin_t * cp;
+  n.vy = vf * sin_t * sp;
+  n.vz = vf * cos_t;
+  n.t = 0;
+  n.p = vf / vi;
+}
+
+}  // namespace mcv

@@ -250,6 +250,10 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
       out.params.push_back(std::sqrt(target[0] * target[0] + target[1] * target[1] + target[2] * target[2]));
       return MCV_OK;
     }
+    case MCV_SRC_IQE_NORMALIZER:  // Ei, Emin, Emax
+      if ((rc = expect(d, i, 3, 0))) return rc;
+      copyp(3);
+      return MCV_OK;
     case MCV_GUIDE:
       if ((rc = expect(d, i, 10, 0))) return rc;
       copyp(10);

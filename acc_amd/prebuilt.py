@@ -109,6 +109,17 @@ def _csg_chain(xmlname, orientation=(0, 0, 0)):
     return i
 
 
+def _iqe_normalizer():
+    from .instrument import instrument
+    from .normalization import IQENormalizerSource
+    from .components.monitors.iqe_monitor import IQE_monitor
+    i = instrument("iqe-normalizer")
+    i.append(IQENormalizerSource("normalizer", 60.0, -50.0, 50.0), position=(0, 0, 0))
+    i.append(IQE_monitor('iqe', Ei=60., Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100),
+             position=(0, 0, 0))
+    return i
+
+
 def instruments():
     from . import configs
     return {
@@ -122,6 +133,7 @@ def instruments():
         "csg-union2": lambda: _csg_chain("union_example2.xml", (90, 0, 0)),
         "csg-intersection": lambda: _csg_chain("intersection_example1.xml", (90, 0, 0)),
         "csg-difference": lambda: _csg_chain("difference_example1.xml", (90, 0, 0)),
+        "iqe-normalizer": _iqe_normalizer,
     }
 
 

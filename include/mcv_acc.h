@@ -83,6 +83,10 @@ extern "C" {
 #define MCV_MON_WAVELENGTH 11
 #define MCV_MON_IQE 12
 #define MCV_DET_TUBES 13
+/* engine-internal sources */
+#define MCV_SRC_IQE_NORMALIZER 20 /* params[3] = Ei, Emin, Emax: the I(Q,E) monitor's
+                                     Monte-Carlo normalizer neutrons
+                                     (acc/components/monitors/iqe_monitor.py:317-344) */
 
 /* scattering kernels of samples (acc/kernels/__init__.py:8-22,363-371) */
 #define MCV_KERNEL_ISOTROPIC 1  /* acc/kernels/isotropic.py:5-21 */

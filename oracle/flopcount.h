@@ -64,6 +64,7 @@ inline orcflops::CReal cos(orcflops::CReal a) { return orcflops::c_trans1(std::c
 inline orcflops::CReal tanh(orcflops::CReal a) { return orcflops::c_trans1(std::tanh(a.v)); }
 inline orcflops::CReal atan(orcflops::CReal a) { return orcflops::c_trans1(std::atan(a.v)); }
 inline orcflops::CReal atan2(orcflops::CReal a, orcflops::CReal b) { return orcflops::c_trans1(std::atan2(a.v, b.v)); }
+inline orcflops::CReal acos(orcflops::CReal a) { return orcflops::c_trans1(std::acos(a.v)); }
 inline orcflops::CReal floor(orcflops::CReal a) { return orcflops::CReal(std::floor(a.v)); }
 inline orcflops::CReal fabs(orcflops::CReal a) { return orcflops::CReal(std::fabs(a.v)); }
 inline bool isnan(orcflops::CReal a) { return std::isnan(a.v); }

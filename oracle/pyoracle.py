@@ -63,6 +63,7 @@ def lib():
         L.orc_component.restype = ctypes.c_int
         L.orc_component.argtypes = [ctypes.POINTER(CompDesc), _dp, _up, _dp, _dp, ctypes.c_int]
         L.orc_partition.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _ip]
+        L.orc_set_trans_nudge.argtypes = [ctypes.c_int]
         L.orc_run.restype = ctypes.c_int64
         L.orc_run.argtypes = [ctypes.POINTER(CompDesc), ctypes.c_int, _dp, _dp, ctypes.c_uint64, ctypes.c_uint64,
                               ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64,
@@ -186,6 +187,11 @@ def partition(N, ntotalthreads=0, tpb=0):
     out = np.zeros(3, dtype=np.int64)
     lib().orc_partition(int(N), int(ntotalthreads), int(tpb), out.ctypes.data_as(_ip))
     return tuple(int(v) for v in out)
+
+
+def set_trans_nudge(ulps):
+    """Move every transcendental result by `ulps` (sign only) ulp: conditioning probe."""
+    lib().orc_set_trans_nudge(int(ulps))
 
 
 def run(descs, offsets, rotmats, N, seed, ntotalthreads=0, tpb=0, gate=True, tid_range=(0, -1), nthreads=0,
