@@ -54,7 +54,42 @@ def chain_body(components):
     return lines
 
 
-def generate_source(instrument):
+def split_index(components):
+    """Index k splitting the chain into phase A = [0..k] and phase B = [k+1..]
+    for the compacted kernel, or None.  Phase B must draw no random numbers
+    (each stream's draws then stay in the reference's order) and must hold a
+    component that is not a monitor (otherwise there is nothing to compact).
+    Phase A ends at the first non-monitor component after the last drawing
+    one: the first point where neutrons die in numbers (a guide entrance)."""
+    from .components.optics.arm import Arm
+    stoch = [i for i, c in enumerate(components) if c.propagate.stochastic]
+    last = stoch[-1] if stoch else -1
+    k = last
+    for j in range(last + 1, len(components)):
+        c = components[j]
+        if not c.propagate.monitor and not isinstance(c, Arm):
+            k = j
+            break
+    if k < 0 or k >= len(components) - 1:
+        return None
+    if all(c.propagate.monitor or isinstance(c, Arm) for c in components[k + 1:]):
+        return None
+    return k
+
+
+def _phase(components, lo, hi, first_transform):
+    lines = []
+    for i in range(lo, hi):
+        comp = components[i]
+        if i > 0 and (i > lo or first_transform):
+            lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
+        lines.append(comp.propagate.render(comp, i))
+        lines.append("if (A.gate && !(n.p > 0)) return 0;")
+    lines.append("return 1;")
+    return "\n  ".join(lines)
+
+
+def generate_source(instrument, compact=True):
     """(source text, structural description) of the fused kernel."""
     comps = instrument.components
     if not comps:
@@ -67,6 +102,33 @@ def generate_source(instrument):
         return generate_ms_source(instrument)
     structure = " -> ".join(type(c).__name__ for c in comps)
     pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
+    k = split_index(comps) if compact else None
+    if k is not None:
+        a = _phase(comps, 0, k + 1, False)
+        b = _phase(comps, k + 1, len(comps), True)
+        src = f"""// fused instrument kernel, compacted after component {k}: {structure}
+#include "mcv_kernel.h"
+
+namespace {{
+{pre}
+__device__ __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)rng; (void)H;
+  {a}
+}}
+
+__device__ __forceinline__ int phase_b(mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)H;
+  {b}
+}}
+}}  // namespace
+
+MCV_DEFINE_COMPACTED_KERNEL(KERNEL_NAME, phase_a, phase_b)
+"""
+        return src, structure + f" [compacted after {k}]"
     body = "\n  ".join(chain_body(comps))
     src = f"""// fused instrument kernel: {structure}
 #include "mcv_kernel.h"
@@ -120,7 +182,7 @@ def compile_source(src, verbose=False):
     return out, name
 
 
-def fuse(instrument, verbose=False):
+def fuse(instrument, verbose=False, compact=True):
     """Compile (or fetch from the cache) the fused kernel of an instrument."""
-    src, _ = generate_source(instrument)
+    src, _ = generate_source(instrument, compact=compact)
     return compile_source(src, verbose=verbose)
