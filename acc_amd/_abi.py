@@ -50,6 +50,8 @@ EXPORTED = (
     "mcv_abi_version", "mcv_last_error", "mcv_device_count", "mcv_instr_create",
     "mcv_instr_set_params", "mcv_instr_run", "mcv_monitor_read", "mcv_monitor_clear",
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
+    "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
+    "mcv_batch_clear", "mcv_batch_destroy",
 )
 
 
@@ -125,6 +127,23 @@ def lib():
     L.mcv_rng_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
     L.mcv_instr_destroy.restype = None
     L.mcv_instr_destroy.argtypes = [ctypes.c_void_p]
+    L.mcv_batch_create.restype = ctypes.c_void_p
+    L.mcv_batch_create.argtypes = [ctypes.POINTER(McvCompDesc), ctypes.c_int32, ctypes.c_int32, dp, dp,
+                                   ctypes.c_char_p, ctypes.c_char_p]
+    L.mcv_batch_run.restype = ctypes.c_int
+    L.mcv_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                                ctypes.POINTER(McvRunStats)]
+    L.mcv_batch_run_sequential.restype = ctypes.c_int
+    L.mcv_batch_run_sequential.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                           ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32,
+                                           ctypes.POINTER(McvRunStats)]
+    L.mcv_batch_monitor_read.restype = ctypes.c_int
+    L.mcv_batch_monitor_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, dp]
+    L.mcv_batch_clear.restype = ctypes.c_int
+    L.mcv_batch_clear.argtypes = [ctypes.c_void_p]
+    L.mcv_batch_destroy.restype = None
+    L.mcv_batch_destroy.argtypes = [ctypes.c_void_p]
     if L.mcv_abi_version() != 1:
         raise ImportError("libmcvacc.so ABI version mismatch")
     _lib = L
@@ -236,3 +255,59 @@ def rng_states(seed, n):
 
 def device_count():
     return lib().mcv_device_count()
+
+
+class BatchHandle:
+    """An mcv_batch: instances of one chain structure, one persistent launch."""
+
+    def __init__(self, descs_per_instance, offsets, rotmats, code_object, kernel_name):
+        L = lib()
+        ninst = len(descs_per_instance)
+        ncomp = len(descs_per_instance[0])
+        flat = [d for inst in descs_per_instance for d in inst]
+        self._descs = CompDescs(flat)
+        self._offsets = np.ascontiguousarray(offsets, dtype=np.float64).reshape(-1)
+        self._rotmats = np.ascontiguousarray(rotmats, dtype=np.float64).reshape(-1)
+        h = L.mcv_batch_create(self._descs.array, ninst, ncomp, _dptr(self._offsets), _dptr(self._rotmats),
+                               code_object.encode(), kernel_name.encode())
+        if not h:
+            raise McvError(MCV_ERR_ARG, L.mcv_last_error().decode())
+        self.h = h
+        self.ninst, self.ncomp = ninst, ncomp
+        self.nbins = [int(d["nbins"]) for d in descs_per_instance[0]]
+
+    def run(self, ncounts, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD, grid=0,
+            use_graph=True):
+        n = np.ascontiguousarray(ncounts, dtype=np.uint64)
+        st = McvRunStats()
+        check(lib().mcv_batch_run(self.h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), int(seed),
+                                  int(ntotalthreads or 0), int(threads_per_block or 0), int(flags), int(grid),
+                                  1 if use_graph else 0, ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_sequential(self, ncounts, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD):
+        n = np.ascontiguousarray(ncounts, dtype=np.uint64)
+        st = McvRunStats()
+        check(lib().mcv_batch_run_sequential(self.h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             int(seed), int(ntotalthreads or 0), int(threads_per_block or 0),
+                                             int(flags), ctypes.byref(st)))
+        return st.as_dict()
+
+    def read_monitor(self, inst, comp):
+        out = np.zeros(3 * self.nbins[comp])
+        check(lib().mcv_batch_monitor_read(self.h, int(inst), int(comp), _dptr(out)))
+        return out.reshape(3, -1)
+
+    def clear(self):
+        check(lib().mcv_batch_clear(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcv_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

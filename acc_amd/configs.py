@@ -10,6 +10,8 @@
    100 x 100 PSD at its exit.
  * c2s: the c2 guides followed by the c1 sample and monitors (the
    source -> guide -> sample -> monitor chain).
+ * scan: 64 instances of a c2s-like chain (Ei and sample rotation scanned,
+   uneven neutron counts) for the batched launch.
 """
 import os
 
@@ -191,6 +193,44 @@ def c3(ntubes=400, npixels=128, ntof=500, max_ms_loops=2, max_ms_loops_path1=3, 
     instr.append(HMS('sample'), position=(0, 0, 10.0), orientation=(90, 0, 0))
     instr.append(tube_array(ntubes=ntubes, npixels=npixels, ntof=ntof), position=(0, 0, 10.0))
     return instr
+
+
+def scan_instance(Ei, rotation_deg, name="scan"):
+    """One instance of the batched scan: the c2s chain with source energy Ei
+    (uniform Ei +- 5%), a 4 x 4 x 0.5 cm isotropic plate sample rotated by
+    `rotation_deg` about the vertical axis (a plate, so the rotation changes
+    the path lengths), an I(Q,E) monitor for that Ei at the sample position
+    (beam frame) and the 250 x 250 PSD."""
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .geometry import Block
+    instr = new_instrument(name)
+    instr.append(_source(Ei, 0.05 * Ei), position=(0, 0, 0))
+    _guides(instr)
+    HSS = factory(shape=Block(0.04, 0.04, 0.005), kernel=isotropic_al_kernel())
+    instr.append(HSS('sample'), position=(0, 0, 30.5), orientation=(0, rotation_deg, 0))
+    instr.append(IQE_monitor('iqe', Ei=Ei, Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100),
+                 position=(0, 0, 30.5))
+    instr.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=250, ny=250), position=(0, 0, 31.5))
+    return instr
+
+
+def scan_plan(ninst=64, seed=0, nmin=1e5, nmax=1e7):
+    """(Ei, rotation, ncount) of each scan instance: Ei log-uniform on
+    20-200 meV, rotation uniform on 0-180 deg, counts log-uniform on
+    [nmin, nmax], all from numpy.random.default_rng(seed)."""
+    g = np.random.default_rng(seed)
+    Ei = np.exp(g.uniform(np.log(20.0), np.log(200.0), ninst))
+    rot = g.uniform(0.0, 180.0, ninst)
+    n = np.exp(g.uniform(np.log(nmin), np.log(nmax), ninst)).astype(np.int64)
+    return Ei, rot, n
+
+
+def scan(ninst=64, seed=0, nmin=1e5, nmax=1e7):
+    """(instruments, ncounts) of the 64-instance scan."""
+    Ei, rot, n = scan_plan(ninst, seed, nmin, nmax)
+    return [scan_instance(float(e), float(r), "scan%d" % k) for k, (e, r) in enumerate(zip(Ei, rot))], n
 
 
 BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3": c3, "c3s": c3_single, "c5s": c5_single}

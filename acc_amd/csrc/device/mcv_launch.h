@@ -29,4 +29,19 @@ struct LaunchArgs {
   int64_t xoff[MCV_MAX_COMP];        // frame change i -> i+1: R[9], o[3]
 };
 
+// Persistent kernel of a batched scan: instances of one chain structure with
+// their own parameter blocks, histograms and neutron counts; all instances
+// share the stream states (same seed, as separate runs would).
+struct BatchArgs {
+  LaunchArgs base;           // offsets, rng, gate; P/H of instance 0
+  int64_t pstride, hstride;  // per-instance strides of the parameter blob and histograms
+  const int64_t* item0;      // (ninst + 1) first work item of each instance (prefix sums)
+  const int64_t* inst_n;     // (ninst) neutrons
+  const int64_t* inst_nthreads;
+  const int64_t* inst_npt;
+  unsigned long long* inst_processed;  // (ninst)
+  unsigned long long* work;  // work-item counter (zeroed before the launch)
+  int32_t ninst;
+};
+
 }  // namespace mcv

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -159,7 +160,7 @@ struct DeviceCtx {
 };
 
 std::mutex g_ctx_mu;
-std::vector<DeviceCtx> g_ctx;
+std::deque<DeviceCtx> g_ctx;  // stable addresses across push_back
 
 int device_ctx(DeviceCtx** out) {
   int dev = 0;
@@ -326,6 +327,56 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
 
 int64_t align8(int64_t n) { return (n + 7) & ~int64_t(7); }
 
+// Pack one instrument: blob = frame changes, then per-component parameters,
+// then data tables (64-byte aligned blocks); offsets go into `args`.
+int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
+                 mcv::LaunchArgs& args, std::vector<double>& B, std::vector<int64_t>& nbins, int64_t& hist_total,
+                 std::vector<int>& types, std::vector<int>& nparams) {
+  std::vector<Packed> packed(ncomp);
+  for (int i = 0; i < ncomp; ++i) {
+    int rc = pack(comps[i], i, packed[i]);
+    if (rc) return rc;
+    types.push_back(comps[i].type);
+    nparams.push_back(comps[i].n_params);
+  }
+  B.clear();
+  for (int i = 0; i + 1 < ncomp; ++i) {
+    args.xoff[i] = (int64_t)B.size();
+    for (int k = 0; k < 9; ++k) B.push_back(rotmats[9 * i + k]);
+    for (int k = 0; k < 3; ++k) B.push_back(offsets[3 * i + k]);
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  for (int i = 0; i < ncomp; ++i) {
+    args.off[i] = (int64_t)B.size();
+    B.insert(B.end(), packed[i].params.begin(), packed[i].params.end());
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  for (int i = 0; i < ncomp; ++i) {
+    args.doff[i] = (int64_t)B.size();
+    B.insert(B.end(), packed[i].data.begin(), packed[i].data.end());
+    B.resize(align8((int64_t)B.size()), 0.0);
+  }
+  if (B.empty()) B.resize(8, 0.0);
+  hist_total = 0;
+  for (int i = 0; i < ncomp; ++i) {
+    args.hoff[i] = hist_total;
+    nbins.push_back(packed[i].nbins);
+    hist_total += 3 * packed[i].nbins;
+  }
+  args.ncomp = ncomp;
+  return MCV_OK;
+}
+
+__global__ void mcv_zero_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void mcv_zero_f64(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 }  // namespace
 
 struct mcv_instr {
@@ -379,39 +430,10 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   mcv_instr* h = new mcv_instr();
   *out = h;
   h->ncomp = ncomp;
-  std::vector<Packed> packed(ncomp);
-  for (int i = 0; i < ncomp; ++i) {
-    int rc = pack(comps[i], i, packed[i]);
-    if (rc) return rc;
-    h->types.push_back(comps[i].type);
-    h->nparams.push_back(comps[i].n_params);
-  }
-  // blob: frame changes, then per-component parameters, then data tables
+  int rc = build_layout(comps, ncomp, offsets, rotmats, h->args, h->blob, h->nbins, h->hist_total, h->types,
+                        h->nparams);
+  if (rc) return rc;
   std::vector<double>& B = h->blob;
-  for (int i = 0; i + 1 < ncomp; ++i) {
-    h->args.xoff[i] = (int64_t)B.size();
-    for (int k = 0; k < 9; ++k) B.push_back(rotmats[9 * i + k]);
-    for (int k = 0; k < 3; ++k) B.push_back(offsets[3 * i + k]);
-    B.resize(align8((int64_t)B.size()), 0.0);
-  }
-  for (int i = 0; i < ncomp; ++i) {
-    h->args.off[i] = (int64_t)B.size();
-    B.insert(B.end(), packed[i].params.begin(), packed[i].params.end());
-    B.resize(align8((int64_t)B.size()), 0.0);
-  }
-  for (int i = 0; i < ncomp; ++i) {
-    h->args.doff[i] = (int64_t)B.size();
-    B.insert(B.end(), packed[i].data.begin(), packed[i].data.end());
-    B.resize(align8((int64_t)B.size()), 0.0);
-  }
-  if (B.empty()) B.resize(8, 0.0);
-  h->hist_total = 0;
-  for (int i = 0; i < ncomp; ++i) {
-    h->args.hoff[i] = h->hist_total;
-    h->nbins.push_back(packed[i].nbins);
-    h->hist_total += 3 * packed[i].nbins;
-  }
-  h->args.ncomp = ncomp;
 
   HIPCHECK(hipStreamCreate(&h->stream));
   for (auto& e : h->ev) HIPCHECK(hipEventCreate(&e));
@@ -594,5 +616,322 @@ int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out) {
 }
 
 void mcv_instr_destroy(mcv_instr* h) { release(h); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batched scans
+struct mcv_batch {
+  int ninst = 0, ncomp = 0;
+  mcv::LaunchArgs args{};
+  int64_t pstride = 0, hstride = 0;
+  std::vector<int64_t> nbins;
+  double* d_blob = nullptr;
+  double* d_hist = nullptr;
+  uint64_t* d_rng = nullptr;
+  int64_t rng_cap = 0;
+  int64_t* d_meta = nullptr;                 // item0 (ninst+1), n, nthreads, npt (ninst each)
+  unsigned long long* d_counters = nullptr;  // processed (ninst), work item counter (1)
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr, fn_batch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<int64_t> graph_key;
+};
+
+namespace {
+
+void release_batch(mcv_batch* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_blob) (void)hipFree(b->d_blob);
+  if (b->d_hist) (void)hipFree(b->d_hist);
+  if (b->d_rng) (void)hipFree(b->d_rng);
+  if (b->d_meta) (void)hipFree(b->d_meta);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->mod) (void)hipModuleUnload(b->mod);
+  delete b;
+}
+
+int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, const double* offsets,
+                      const double* rotmats, const char* code_object, const char* kernel_name, mcv_batch** out) {
+  if (!comps || ninst < 1 || ncomp < 1) return fail(MCV_ERR_ARG, "empty batch");
+  if (ncomp > MCV_MAX_COMP) return fail(MCV_ERR_ARG, "chain longer than %d components", MCV_MAX_COMP);
+  if (ncomp > 1 && (!offsets || !rotmats)) return fail(MCV_ERR_ARG, "missing frame transformations");
+  if (!code_object || !kernel_name) return fail(MCV_ERR_ARG, "missing code object / kernel name");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MCV_ERR_HIP, "no HIP device available");
+  mcv_batch* b = new mcv_batch();
+  *out = b;
+  b->ninst = ninst;
+  b->ncomp = ncomp;
+  std::vector<double> all;
+  int64_t hist_total = 0;
+  for (int k = 0; k < ninst; ++k) {
+    mcv::LaunchArgs a{};
+    std::vector<double> blob;
+    std::vector<int64_t> nb;
+    std::vector<int> types, np;
+    int64_t ht = 0;
+    int rc = build_layout(comps + (size_t)k * ncomp, ncomp, offsets + (size_t)k * (ncomp - 1) * 3,
+                          rotmats + (size_t)k * (ncomp - 1) * 9, a, blob, nb, ht, types, np);
+    if (rc) return rc;
+    if (k == 0) {
+      b->args = a;
+      b->nbins = nb;
+      b->pstride = (int64_t)blob.size();
+      hist_total = ht;
+    } else if ((int64_t)blob.size() != b->pstride || ht != hist_total ||
+               std::memcmp(a.off, b->args.off, sizeof(a.off)) || std::memcmp(a.doff, b->args.doff, sizeof(a.doff)) ||
+               std::memcmp(a.hoff, b->args.hoff, sizeof(a.hoff))) {
+      return fail(MCV_ERR_ARG, "instance %d differs in layout from instance 0 (tables and histograms must match)", k);
+    }
+    all.insert(all.end(), blob.begin(), blob.end());
+  }
+  b->hstride = hist_total;
+  HIPCHECK(hipStreamCreate(&b->stream));
+  for (auto& e : b->ev) HIPCHECK(hipEventCreate(&e));
+  HIPCHECK(hipMalloc(&b->d_blob, all.size() * sizeof(double)));
+  HIPCHECK(hipMemcpy(b->d_blob, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+  const int64_t hn = std::max<int64_t>(hist_total * ninst, 1);
+  HIPCHECK(hipMalloc(&b->d_hist, hn * sizeof(double)));
+  HIPCHECK(hipMemset(b->d_hist, 0, hn * sizeof(double)));
+  HIPCHECK(hipMalloc(&b->d_meta, (size_t)(4 * ninst + 1) * sizeof(int64_t)));
+  HIPCHECK(hipMalloc(&b->d_counters, (size_t)(ninst + 1) * sizeof(unsigned long long)));
+  hipError_t e = hipModuleLoad(&b->mod, code_object);
+  if (e != hipSuccess) return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
+  if (hipModuleGetFunction(&b->fn, b->mod, kernel_name) != hipSuccess)
+    return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
+  std::string bn = std::string(kernel_name) + "_batch";
+  if (hipModuleGetFunction(&b->fn_batch, b->mod, bn.c_str()) != hipSuccess)
+    return fail(MCV_ERR_MODULE, "kernel %s not in %s", bn.c_str(), code_object);
+  return MCV_OK;
+}
+
+int ensure_rng(mcv_batch* b, int64_t n) {
+  if (n <= b->rng_cap) return MCV_OK;
+  if (b->d_rng) HIPCHECK(hipFree(b->d_rng));
+  b->d_rng = nullptr;
+  HIPCHECK(hipMalloc(&b->d_rng, (size_t)n * 2 * sizeof(uint64_t)));
+  b->rng_cap = n;
+  if (b->gexec) {  // the graph captured the old buffer
+    (void)hipGraphExecDestroy(b->gexec);
+    (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+  }
+  return MCV_OK;
+}
+
+// stream seeding + counter reset + persistent batch kernel, on b->stream
+int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t total_items, uint32_t flags,
+                  int32_t grid) {
+  int rc = seed_streams(b->stream, seed, nthreads_max, b->d_rng);
+  if (rc) return rc;
+  hipLaunchKernelGGL(mcv_zero_u64, dim3((unsigned)((b->ninst + 1 + 255) / 256)), dim3(256), 0, b->stream,
+                     b->d_counters, (int64_t)(b->ninst + 1));
+  HIPCHECK(hipGetLastError());
+  mcv::BatchArgs B{};
+  B.base = b->args;
+  B.base.P = b->d_blob;
+  B.base.H = b->d_hist;
+  B.base.rng = b->d_rng;
+  B.base.dump = nullptr;
+  B.base.dump_alive = nullptr;
+  B.base.dump_count = 0;
+  B.base.tid_base = 0;
+  B.base.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+  B.pstride = b->pstride;
+  B.hstride = b->hstride;
+  B.item0 = b->d_meta;
+  B.inst_n = b->d_meta + b->ninst + 1;
+  B.inst_nthreads = b->d_meta + 2 * b->ninst + 1;
+  B.inst_npt = b->d_meta + 3 * b->ninst + 1;
+  B.inst_processed = b->d_counters;
+  B.work = b->d_counters + b->ninst;
+  B.ninst = b->ninst;
+  const int64_t g = std::min<int64_t>(grid > 0 ? grid : 1024, std::max<int64_t>(total_items, 1));
+  void* kargs[] = {&B};
+  HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, 256, 1, 1, 0, b->stream, kargs, nullptr));
+  return MCV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mcv_batch* mcv_batch_create(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, const double* offsets,
+                            const double* rotmats, const char* code_object, const char* kernel_name) {
+  mcv_batch* b = nullptr;
+  int rc = batch_create_impl(comps, ninst, ncomp, offsets, rotmats, code_object, kernel_name, &b);
+  if (rc) {
+    std::string msg = g_last_error;
+    release_batch(b);
+    g_last_error = msg;
+    return nullptr;
+  }
+  return b;
+}
+
+int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t ntotalthreads, int32_t tpb,
+                  uint32_t flags, int32_t grid, int32_t use_graph, mcv_run_stats* stats) {
+  if (!b || !ncounts) return fail(MCV_ERR_ARG, "bad batch arguments");
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = b->ninst;
+  std::vector<int64_t> meta(4 * n + 1, 0);
+  int64_t nthreads_max = 0;
+  uint64_t total_n = 0;
+  for (int k = 0; k < n; ++k) {
+    int64_t nb, nt, npt;
+    partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
+    meta[k + 1] = meta[k] + (nt + 255) / 256;
+    meta[n + 1 + k] = (int64_t)ncounts[k];
+    meta[2 * n + 1 + k] = nt;
+    meta[3 * n + 1 + k] = npt;
+    nthreads_max = std::max(nthreads_max, nt);
+    total_n += ncounts[k];
+  }
+  const int64_t total_items = meta[n];
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->nthreads = nthreads_max;
+  }
+  int rc = ensure_rng(b, std::max<int64_t>(nthreads_max, 1));
+  if (rc) return rc;
+  HIPCHECK(hipMemcpyAsync(b->d_meta, meta.data(), meta.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+  HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+  if (total_items > 0) {
+    if (use_graph) {
+      std::vector<int64_t> key = {(int64_t)seed, nthreads_max, total_items, (int64_t)flags, grid};
+      if (!b->gexec || key != b->graph_key) {
+        if (b->gexec) {
+          (void)hipGraphExecDestroy(b->gexec);
+          (void)hipGraphDestroy(b->graph);
+          b->gexec = nullptr;
+          b->graph = nullptr;
+        }
+        HIPCHECK(hipStreamSynchronize(b->stream));
+        DeviceCtx* ctx = nullptr;  // allocate the jump tables before capture, not inside it
+        rc = device_ctx(&ctx);
+        if (rc) return rc;
+        HIPCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_batch(b, seed, nthreads_max, total_items, flags, grid);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess) return fail(MCV_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        b->graph = g;
+        HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->graph_key = key;
+        HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+      }
+      HIPCHECK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+      rc = enqueue_batch(b, seed, nthreads_max, total_items, flags, grid);
+      if (rc) return rc;
+    }
+  }
+  HIPCHECK(hipEventRecord(b->ev[1], b->stream));
+  HIPCHECK(hipStreamSynchronize(b->stream));
+  std::vector<unsigned long long> processed(n + 1, 0);
+  HIPCHECK(hipMemcpy(processed.data(), b->d_counters, (n + 1) * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  uint64_t done = 0;
+  for (int k = 0; k < n; ++k) done += total_items > 0 ? processed[k] : 0;
+  if (stats) {
+    stats->processed = done;
+    stats->ms_kernel = ms;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  for (int k = 0; k < n; ++k)
+    if (total_items > 0 && processed[k] != ncounts[k])
+      return fail(MCV_ERR_COUNT, "instance %d: processed %llu of %llu neutrons", k,
+                  (unsigned long long)processed[k], (unsigned long long)ncounts[k]);
+  return MCV_OK;
+}
+
+int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t ntotalthreads,
+                             int32_t tpb, uint32_t flags, mcv_run_stats* stats) {
+  if (!b || !ncounts) return fail(MCV_ERR_ARG, "bad batch arguments");
+  const auto t0 = std::chrono::steady_clock::now();
+  double ms_sum = 0;
+  uint64_t done = 0;
+  for (int k = 0; k < b->ninst; ++k) {
+    if (ncounts[k] == 0) continue;
+    int64_t nb, nt, npt;
+    partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
+    int rc = ensure_rng(b, nt);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(b->ev[0], b->stream));
+    rc = seed_streams(b->stream, seed, nt, b->d_rng);
+    if (rc) return rc;
+    hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, b->stream, b->d_counters, (int64_t)1);
+    mcv::LaunchArgs A = b->args;
+    A.P = b->d_blob + k * b->pstride;
+    A.H = b->d_hist + k * b->hstride;
+    A.rng = b->d_rng;
+    A.counter = b->d_counters;
+    A.dump = nullptr;
+    A.dump_alive = nullptr;
+    A.dump_count = 0;
+    A.N = (int64_t)ncounts[k];
+    A.nthreads = nt;
+    A.npt = npt;
+    A.tid_base = 0;
+    A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+    void* kargs[] = {&A};
+    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + 255) / 256), 1, 1, 256, 1, 1, 0, b->stream, kargs,
+                                   nullptr));
+    HIPCHECK(hipEventRecord(b->ev[1], b->stream));
+    HIPCHECK(hipStreamSynchronize(b->stream));
+    unsigned long long processed = 0;
+    HIPCHECK(hipMemcpy(&processed, b->d_counters, sizeof(processed), hipMemcpyDeviceToHost));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    ms_sum += ms;
+    done += processed;
+    if (processed != ncounts[k])
+      return fail(MCV_ERR_COUNT, "instance %d: processed %llu of %llu neutrons", k, processed,
+                  (unsigned long long)ncounts[k]);
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->processed = done;
+    stats->ms_kernel = ms_sum;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return MCV_OK;
+}
+
+int mcv_batch_monitor_read(mcv_batch* b, int32_t inst, int32_t comp, double* out3) {
+  if (!b || inst < 0 || inst >= b->ninst || comp < 0 || comp >= b->ncomp)
+    return fail(MCV_ERR_ARG, "bad batch, instance or component index");
+  if (b->nbins[comp] <= 0) return fail(MCV_ERR_ARG, "component %d is not a monitor", comp);
+  HIPCHECK(hipMemcpy(out3, b->d_hist + inst * b->hstride + b->args.hoff[comp], 3 * b->nbins[comp] * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return MCV_OK;
+}
+
+int mcv_batch_clear(mcv_batch* b) {
+  if (!b) return fail(MCV_ERR_ARG, "null batch");
+  const int64_t n = b->hstride * b->ninst;
+  if (n > 0) {
+    hipLaunchKernelGGL(mcv_zero_f64, dim3(1024), dim3(256), 0, b->stream, b->d_hist, n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(b->stream));
+  }
+  return MCV_OK;
+}
+
+void mcv_batch_destroy(mcv_batch* b) { release_batch(b); }
 
 }  // extern "C"

@@ -16,6 +16,7 @@ import hashlib
 import os
 import subprocess
 import tempfile
+import threading
 
 _here = os.path.dirname(os.path.abspath(__file__))
 DEVICE_DIR = os.path.join(_here, "csrc", "device")
@@ -49,8 +50,8 @@ def chain_body(components):
         if i > 0:
             lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
         lines.append(comp.propagate.render(comp, i))
-        lines.append("if (A.gate && !(n.p > 0)) return false;")
-    lines.append("return true;")
+        lines.append("if (A.gate && !(n.p > 0)) return 0;")
+    lines.append("return 1;")
     return lines
 
 
@@ -111,22 +112,24 @@ def generate_source(instrument, compact=True):
 
 namespace {{
 {pre}
-__device__ __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+struct Chain {{
+  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
   (void)rng; (void)H;
   {a}
-}}
+  }}
 
-__device__ __forceinline__ int phase_b(mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int phase_b(mcv::Neutron& n, const mcv::LaunchArgs& A) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
   (void)H;
   {b}
-}}
+  }}
+}};
 }}  // namespace
 
-MCV_DEFINE_COMPACTED_KERNEL(KERNEL_NAME, phase_a, phase_b)
+MCV_DEFINE_KERNELS_COMPACTED(KERNEL_NAME, Chain)
 """
         return src, structure + f" [compacted after {k}]"
     body = "\n  ".join(chain_body(comps))
@@ -135,15 +138,17 @@ MCV_DEFINE_COMPACTED_KERNEL(KERNEL_NAME, phase_a, phase_b)
 
 namespace {{
 {pre}
-__device__ __forceinline__ bool chain_body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+struct Chain {{
+  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
   (void)rng; (void)H;
   {body}
-}}
+  }}
+}};
 }}  // namespace
 
-MCV_DEFINE_FUSED_KERNEL(KERNEL_NAME, chain_body)
+MCV_DEFINE_KERNELS_PLAIN(KERNEL_NAME, Chain)
 """
     return src, structure
 
@@ -169,7 +174,7 @@ def compile_source(src, verbose=False):
         srcpath = os.path.join(td, name + ".hip")
         with open(srcpath, "w") as f:
             f.write(text)
-        tmp_out = out + f".tmp{os.getpid()}"
+        tmp_out = out + f".tmp{os.getpid()}_{threading.get_ident()}"
         cmd = [HIPCC] + HIPCC_FLAGS + ["-I", DEVICE_DIR, srcpath, "-o", tmp_out]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -52,14 +52,16 @@ def generate_ms_source(instrument):
 
 namespace {{
 {pre}
-__device__ __forceinline__ int chain_body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+struct Chain {{
+  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
   (void)rng; (void)H;
   {body}
-}}
+  }}
+}};
 }}  // namespace
 
-MCV_DEFINE_FUSED_KERNEL(KERNEL_NAME, chain_body)
+MCV_DEFINE_KERNELS_PLAIN(KERNEL_NAME, Chain)
 """
     return src, structure

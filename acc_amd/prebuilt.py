@@ -134,12 +134,17 @@ def instruments():
         "csg-intersection": lambda: _csg_chain("intersection_example1.xml", (90, 0, 0)),
         "csg-difference": lambda: _csg_chain("difference_example1.xml", (90, 0, 0)),
         "iqe-normalizer": _iqe_normalizer,
+        "scan": lambda: configs.scan_instance(60.0, 30.0),
     }
 
 
-def build_all(verbose=False):
-    from .fuser import fuse
-    out = {}
-    for name, make in instruments().items():
-        out[name] = fuse(make(), verbose=verbose)
-    return out
+def build_all(verbose=False, jobs=None):
+    """Compile every listed chain (hipcc runs in parallel, one per job)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from .fuser import generate_source, compile_source
+    srcs = {name: generate_source(make())[0] for name, make in instruments().items()}
+    unique = sorted(set(srcs.values()))
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        built = dict(zip(unique, ex.map(lambda s: compile_source(s, verbose=verbose), unique)))
+    return {name: built[s] for name, s in srcs.items()}

@@ -186,6 +186,37 @@ int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out);
 /* Release the handle and its device memory. */
 void mcv_instr_destroy(mcv_instr* h);
 
+/* ---- batched scans ----------------------------------------------------------
+ * No reference counterpart: the reference runs one instrument per
+ * run_script.run call (acc/run_script.py:10-35), so a parameter scan is a loop
+ * of separate launches.  A batch holds `ninst` instances of one chain structure
+ * (comps = ninst x ncomp descriptors, instance-major; offsets/rotmats per
+ * instance) with their own parameters and histograms; tables and histogram
+ * sizes must match across instances. */
+typedef struct mcv_batch mcv_batch;
+
+mcv_batch* mcv_batch_create(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp,
+                            const double* offsets, const double* rotmats,
+                            const char* code_object, const char* kernel_name);
+
+/* All instances in one persistent launch (kernel `<kernel_name>_batch`):
+ * workgroups take (instance, block of 256 streams) items from one atomic
+ * counter.  Instance i runs ncounts[i] neutrons with the same partition and
+ * streams as a separate mcv_instr_run with the same seed would.  grid <= 0
+ * selects 1024 workgroups; use_graph != 0 captures seeding + launch in a
+ * hipGraph and replays it. */
+int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t ntotalthreads,
+                  int32_t tpb, uint32_t flags, int32_t grid, int32_t use_graph, mcv_run_stats* stats);
+
+/* The same instances as one launch each, in order (the loop of separate
+ * runs a scan would otherwise be): the comparison point of mcv_batch_run. */
+int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t seed,
+                             int64_t ntotalthreads, int32_t tpb, uint32_t flags, mcv_run_stats* stats);
+
+int mcv_batch_monitor_read(mcv_batch* b, int32_t inst, int32_t comp, double* out3);
+int mcv_batch_clear(mcv_batch* b);
+void mcv_batch_destroy(mcv_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

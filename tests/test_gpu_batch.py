@@ -1,0 +1,103 @@
+"""Batched scans: one persistent launch over many instances of one chain gives
+each instance the histograms of its own one-by-one run.
+
+Instance k with ncounts[k] neutrons uses the partition and streams of a
+separate run with the same seed, so per-bin neutron counts must be equal and
+weight sums equal up to the order of atomic additions.  The one-by-one runs
+here go through `run_script.run_instrument` (the reference's single-run path,
+acc/run_script.py:10-35), which is checked against the oracle by the parity
+tests."""
+import numpy as np
+import pytest
+
+from acc_amd import configs, run_script
+from acc_amd.batch import BatchedScan
+
+pytestmark = pytest.mark.gpu
+
+SEED = 5
+# a few instances with uneven counts, one of them empty
+PLAN = [(25.0, 0.0, 30000), (60.0, 45.0, 250000), (120.0, 90.0, 0), (180.0, 170.0, 1300), (40.0, 10.0, 70001)]
+
+
+def _instances():
+    return [configs.scan_instance(e, r, "scan%d" % k) for k, (e, r, _) in enumerate(PLAN)]
+
+
+def _expected():
+    out = []
+    for k, (e, r, n) in enumerate(PLAN):
+        instr = configs.scan_instance(e, r, "single%d" % k)
+        if n:
+            run_script.run_instrument(instr, n, seed=SEED, accumulate=False)
+            h = run_script.get_handle(instr)
+            out.append({i: h.read_monitor(i) for i, c in enumerate(instr.components) if h.nbins[i] > 0})
+        else:
+            out.append(None)
+    return out
+
+
+def _check(scan, expected):
+    for k, exp in enumerate(expected):
+        for i, nb in enumerate(scan.handle.nbins):
+            if nb <= 0:
+                continue
+            got = scan.monitor(k, i)
+            if exp is None:
+                assert not got.any(), "instance %d (no neutrons) has counts" % k
+                continue
+            np.testing.assert_array_equal(got[0], exp[i][0], err_msg="instance %d comp %d counts" % (k, i))
+            np.testing.assert_allclose(got[1:], exp[i][1:], rtol=1e-9, atol=1e-300,
+                                       err_msg="instance %d comp %d sums" % (k, i))
+
+
+@pytest.fixture(scope="module")
+def scan_and_expected():
+    print("batch: building", flush=True)
+    scan = BatchedScan(_instances())
+    exp = _expected()
+    yield scan, exp
+    scan.close()
+
+
+def test_batch_matches_single_runs(scan_and_expected):
+    scan, exp = scan_and_expected
+    n = [p[2] for p in PLAN]
+    st = scan.run(n, seed=SEED, use_graph=False)
+    assert st["processed"] == sum(n)
+    _check(scan, exp)
+
+
+def test_batch_graph_replays(scan_and_expected):
+    scan, exp = scan_and_expected
+    n = [p[2] for p in PLAN]
+    for rep in range(3):  # capture, then two replays
+        st = scan.run(n, seed=SEED, use_graph=True, grid=96)
+        assert st["processed"] == sum(n)
+        _check(scan, exp)
+
+
+def test_batch_sequential_matches(scan_and_expected):
+    scan, exp = scan_and_expected
+    n = [p[2] for p in PLAN]
+    st = scan.run_sequential(n, seed=SEED)
+    assert st["processed"] == sum(n)
+    _check(scan, exp)
+
+
+def test_batch_new_counts_recapture(scan_and_expected):
+    """Changing the counts after a capture re-captures; an all-zero scan runs nothing."""
+    scan, _ = scan_and_expected
+    n = [1000, 0, 0, 0, 5]
+    st = scan.run(n, seed=SEED, use_graph=True)
+    assert st["processed"] == sum(n)
+    st = scan.run([0] * len(PLAN), seed=SEED, use_graph=True)
+    assert st["processed"] == 0
+    for k in range(len(PLAN)):
+        assert not scan.monitor(k, "psd").any()
+
+
+def test_batch_rejects_different_structure():
+    from acc_amd.prebuilt import instruments
+    with pytest.raises(ValueError):
+        BatchedScan([configs.scan_instance(60.0, 0.0), instruments()["c2s"]()])
