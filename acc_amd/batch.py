@@ -6,8 +6,9 @@ parameter is a Python loop of separate runs, each a full kernel launch that
 leaves the GPU mostly idle when its neutron count is small.  `BatchedScan`
 takes instruments that share one chain structure (same component types,
 shapes, kernels, table sizes and histogram sizes; any parameter values) and
-runs them all in one persistent kernel: workgroups pull (instance, block of
-256 streams) work items from one atomic counter, so small and large instances
+runs them all in one persistent kernel: workgroups pull (instance, one
+workgroup's worth of streams) work items from one atomic counter, so small and
+large instances
 share the chip.  Instance k with ncounts[k] neutrons uses exactly the
 partition and random streams a separate `run_instrument(instr_k, ncounts[k],
 seed=seed)` would, so each instance's histograms are identical (up to the
@@ -25,11 +26,11 @@ from .instrument import calcTransformations
 class BatchedScan:
     """Instances of one chain structure, run together on the GPU."""
 
-    def __init__(self, instruments, compact=True):
+    def __init__(self, instruments, compact=True, block=None):
         from .fuser import generate_source, compile_source
         if not instruments:
             raise ValueError("empty scan")
-        srcs = [generate_source(instr, compact=compact)[0] for instr in instruments]
+        srcs = [generate_source(instr, compact=compact, block=block)[0] for instr in instruments]
         for k, s in enumerate(srcs):
             if s != srcs[0]:
                 raise ValueError("instance %d has a different chain structure than instance 0" % k)

@@ -6,10 +6,14 @@
 neutron once its weight p <= 0 (the mcvine engine's, and the reference's
 buffered mode, acc/components/ComponentBase.py:219-221); False reproduces the
 reference fused kernel, where an absorbed neutron keeps flowing downstream
-(acc/run_script.py:182-196, acc/neutron.py:48-58).
+(acc/run_script.py:182-196, acc/neutron.py:48-58).  `kernel_block` is the
+GPU workgroup size the fuser compiles for.
 """
 ntotalthreads = int(1e6)
 threads_per_block = 512
+# workgroup size of the fused kernels (results do not depend on it; the
+# reference's threads_per_block only shapes the stream partition)
+kernel_block = 256
 floattype = "float64"
 rng_seed = 1
 gate_dead_neutrons = True

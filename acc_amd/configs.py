@@ -195,20 +195,22 @@ def c3(ntubes=400, npixels=128, ntof=500, max_ms_loops=2, max_ms_loops_path1=3, 
     return instr
 
 
-def scan_instance(Ei, rotation_deg, name="scan"):
+def scan_instance(Ei, rotation_deg, name="scan", sample="sphere"):
     """One instance of the batched scan: the c2s chain with source energy Ei
-    (uniform Ei +- 5%), a 4 x 4 x 0.5 cm isotropic plate sample rotated by
-    `rotation_deg` about the vertical axis (a plate, so the rotation changes
-    the path lengths), an I(Q,E) monitor for that Ei at the sample position
-    (beam frame) and the 250 x 250 PSD."""
+    (uniform Ei +- 5%), the c1 isotropic sample (sample="sphere"; or a
+    4 x 4 x 0.5 cm plate, sample="plate", whose path lengths do change with
+    the rotation) rotated by `rotation_deg` about the vertical axis, an I(Q,E)
+    monitor for that Ei at the sample position (beam frame) and the
+    250 x 250 PSD."""
     from .components.samples.homogeneous_single_scatterer import factory
     from .components.monitors.psd_monitor import PSD_monitor
     from .components.monitors.iqe_monitor import IQE_monitor
-    from .geometry import Block
+    from .geometry import Block, Sphere
     instr = new_instrument(name)
     instr.append(_source(Ei, 0.05 * Ei), position=(0, 0, 0))
     _guides(instr)
-    HSS = factory(shape=Block(0.04, 0.04, 0.005), kernel=isotropic_al_kernel())
+    shape = {"sphere": lambda: Sphere(0.01), "plate": lambda: Block(0.04, 0.04, 0.005)}[sample]()
+    HSS = factory(shape=shape, kernel=isotropic_al_kernel())
     instr.append(HSS('sample'), position=(0, 0, 30.5), orientation=(0, rotation_deg, 0))
     instr.append(IQE_monitor('iqe', Ei=Ei, Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100),
                  position=(0, 0, 30.5))
@@ -227,10 +229,11 @@ def scan_plan(ninst=64, seed=0, nmin=1e5, nmax=1e7):
     return Ei, rot, n
 
 
-def scan(ninst=64, seed=0, nmin=1e5, nmax=1e7):
+def scan(ninst=64, seed=0, nmin=1e5, nmax=1e7, sample="sphere"):
     """(instruments, ncounts) of the 64-instance scan."""
     Ei, rot, n = scan_plan(ninst, seed, nmin, nmax)
-    return [scan_instance(float(e), float(r), "scan%d" % k) for k, (e, r) in enumerate(zip(Ei, rot))], n
+    return [scan_instance(float(e), float(r), "scan%d" % k, sample)
+            for k, (e, r) in enumerate(zip(Ei, rot))], n
 
 
 BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3": c3, "c3s": c3_single, "c5s": c5_single}

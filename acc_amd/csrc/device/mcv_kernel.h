@@ -12,7 +12,7 @@
 //   NAME        one instrument, one launch over its streams;
 //   NAME_batch  a persistent kernel for a scan of many instances of the same
 //               chain (own parameters, histograms and neutron counts each):
-//               workgroups pull (instance, block of 256 streams) items from
+//               workgroups pull (instance, block of MCV_BLOCK streams) items from
 //               one atomic counter until all are done, so uneven instances
 //               share the GPU instead of queueing one launch after another.
 #pragma once
@@ -21,8 +21,17 @@
 #include "mcv_samples.h"
 #include "mcv_launch.h"
 
+// Workgroup size of the chain kernels, fixed per code object (the fuser passes
+// -DMCV_BLOCK=...); the runtime reads it back from `mcv_block_size`.
+#ifndef MCV_BLOCK
 #define MCV_BLOCK 256
-#define MCV_QCAP 512  // >= (MCV_BLOCK - 1) + MCV_BLOCK
+#endif
+#if MCV_BLOCK % 64 != 0 || MCV_BLOCK < 64 || MCV_BLOCK > 1024
+#error "MCV_BLOCK must be a multiple of 64 in [64, 1024]"
+#endif
+#define MCV_QCAP (2 * MCV_BLOCK)  // >= (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
+
+extern "C" __device__ int mcv_block_size = MCV_BLOCK;
 
 namespace mcv {
 
@@ -77,34 +86,87 @@ __device__ __forceinline__ void run_lane(const LaunchArgs& A, int64_t tid) {
 }
 
 // ------------------------------------------------------------------ compacted chains
-// A chain whose random draws all happen in its first components (phase A) and
-// whose remaining components (phase B) are deterministic runs in two phases per
-// workgroup.  Every lane produces neutrons from its own stream, in stream
-// order, through phase A; survivors are appended to an LDS queue (slots from a
-// wavefront ballot + prefix count, one LDS atomic per wave); whenever the queue
-// holds a full workgroup of neutrons, every lane pops one and runs phase B.
-// Phase B draws nothing, so running it later keeps each stream's draw sequence
-// identical to the reference's neutron-by-neutron loop, and its expensive part
-// (e.g. the segmented guide) runs with all lanes busy instead of one in four.
-struct QueueLDS {
-  double x[MCV_QCAP], y[MCV_QCAP], z[MCV_QCAP], vx[MCV_QCAP], vy[MCV_QCAP], vz[MCV_QCAP], t[MCV_QCAP],
-      p[MCV_QCAP];
-  int64_t idx[MCV_QCAP];
-  int count;
+// A chain whose components after its first culling point (the first
+// component after the source that drops neutrons in numbers, e.g. a guide
+// entrance) are expensive runs in two phases per workgroup.  Every lane
+// produces neutrons from its own stream, in stream order, through phase A;
+// survivors are appended to an LDS queue (slots from a wavefront ballot +
+// prefix count, one LDS atomic per wave), and phase B runs over the queue with
+// all lanes of whole wavefronts busy instead of one in four.
+//
+// Control decisions need the workgroup-wide queue length and whether any lane
+// still has work; each wave publishes its two counts in a per-round slot
+// (double-buffered by round parity, waves are never more than one round
+// apart), so one barrier per round makes every lane see the same values.
+template <int CAP>
+struct QueueCore {
+  double x[CAP], y[CAP], z[CAP], vx[CAP], vy[CAP], vz[CAP], t[CAP], p[CAP];
+  int64_t idx[CAP];
+  int count;  // slot allocator
+  int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
 };
 
-__device__ __forceinline__ void q_store(QueueLDS& q, int s, const Neutron& n, int64_t idx) {
+// deterministic phase B: up to (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
+struct QueueLDS : QueueCore<MCV_QCAP> {};
+
+// drawing phase B: at most one queued neutron per lane, with its stream state
+struct QueueRngLDS : QueueCore<MCV_BLOCK> {
+  uint64_t s0[MCV_BLOCK], s1[MCV_BLOCK];  // stream state carried with the neutron
+  int owner[MCV_BLOCK];
+  uint64_t r0[MCV_BLOCK], r1[MCV_BLOCK];  // stream state handed back to its owner lane
+  int returned[MCV_BLOCK];
+};
+
+template <class Q>
+__device__ __forceinline__ void q_store(Q& q, int s, const Neutron& n, int64_t idx) {
   q.x[s] = n.x; q.y[s] = n.y; q.z[s] = n.z; q.vx[s] = n.vx; q.vy[s] = n.vy; q.vz[s] = n.vz;
   q.t[s] = n.t; q.p[s] = n.p; q.idx[s] = idx;
 }
 
-__device__ __forceinline__ int64_t q_load(const QueueLDS& q, int s, Neutron& n) {
+template <class Q>
+__device__ __forceinline__ int64_t q_load(const Q& q, int s, Neutron& n) {
   n.x = q.x[s]; n.y = q.y[s]; n.z = q.z[s]; n.vx = q.vx[s]; n.vy = q.vy[s]; n.vz = q.vz[s];
   n.t = q.t[s]; n.p = q.p[s]; n.s1 = 0; n.s2 = 0;
   return q.idx[s];
 }
 
-// All MCV_BLOCK threads of the workgroup must call this together (barriers).
+// Append `has` lanes' neutrons; returns the slot of this lane (or -1).
+template <class Q>
+__device__ __forceinline__ int q_append(Q& q, bool has) {
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned long long m = __ballot(has);
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(&q.count, (int)__popcll(m));
+  base = __shfl(base, 0, 64);
+  return has ? base + (int)__popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Barrier; every lane gets the workgroup totals of `has` (neutrons appended
+// this round) and whether any lane has `more`.
+template <class Q>
+__device__ __forceinline__ void round_totals(Q& q, int& par, bool has, bool more, int& n_new,
+                                             bool& any_more) {
+  const unsigned long long mh = __ballot(has), mm = __ballot(more);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    q.wave_new[par][w] = (int)__popcll(mh);
+    q.wave_more[par][w] = mm != 0ull;
+  }
+  __syncthreads();
+  n_new = 0;
+  int mo = 0;
+#pragma unroll
+  for (int k = 0; k < MCV_BLOCK / 64; ++k) {
+    n_new += q.wave_new[par][k];
+    mo |= q.wave_more[par][k];
+  }
+  any_more = mo != 0;
+  par ^= 1;
+}
+
+// Phase B draws nothing: each stream runs ahead through phase A while its
+// earlier neutrons wait in the queue, and the draw order per stream is the
+// reference's.  All MCV_BLOCK threads must call this together (barriers).
 template <class Chain>
 __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t tid, QueueLDS& q) {
   const int lt = threadIdx.x;
@@ -113,10 +175,9 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
   int64_t idx, end;
   stream_range(A, tid, rng, idx, end);
   const int64_t mine = end - idx;
-  const unsigned lane = threadIdx.x & 63u;
+  int par = 0, cnt = 0;
   __syncthreads();
   while (true) {
-    // phase A: at most one neutron per lane per round
     bool has = false;
     Neutron n;
     n.s1 = 0;
@@ -127,13 +188,12 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
       if (!has) dump_neutron(A, my, n, 0);
       ++idx;
     }
-    const unsigned long long m = __ballot(has);
-    int base = 0;
-    if (lane == 0 && m) base = atomicAdd(&q.count, (int)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (has) q_store(q, base + (int)__popcll(m & ((1ull << lane) - 1ull)), n, my);
-    const int more = __syncthreads_or(idx < end);
-    const int cnt = q.count;
+    const int s = q_append(q, has);
+    if (has) q_store(q, s, n, my);
+    int n_new;
+    bool more;
+    round_totals(q, par, has, idx < end, n_new, more);
+    cnt += n_new;
     if (cnt >= MCV_BLOCK || (!more && cnt > 0)) {
       const int take = cnt < MCV_BLOCK ? cnt : MCV_BLOCK;
       if (lt < take) {
@@ -142,16 +202,91 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
         const int alive = (int)Chain::phase_b(b, A);
         dump_neutron(A, bidx, b, alive);
       }
+      cnt -= take;
       __syncthreads();
-      if (lt == 0) q.count = cnt - take;
+      if (lt == 0) q.count = cnt;
       __syncthreads();
-      if (!more && cnt - take == 0) break;
     } else if (!more) {
       break;
     }
   }
   count_processed(A.counter, mine);
   __syncthreads();  // the queue is reused by the next block of streams
+}
+
+// Phase B draws too: a surviving neutron carries its stream's state into the
+// queue and its owner lane pauses that stream until phase B hands the state
+// back, so every stream still draws for its neutrons one after another in
+// the reference's order.  Phase B runs over whole wavefronts of queued
+// neutrons (leftovers wait for the next flush) once half a workgroup is
+// queued, or over everything when no lane can run phase A.
+template <class Chain>
+__device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q) {
+  const int lt = threadIdx.x;
+  if (lt == 0) q.count = 0;
+  q.returned[lt] = 0;
+  Rng rng;
+  int64_t idx, end;
+  stream_range(A, tid, rng, idx, end);
+  const int64_t mine = end - idx;
+  int par = 0, cnt = 0;
+  bool busy = false;
+  __syncthreads();
+  while (true) {
+    bool has = false;
+    Neutron n;
+    n.s1 = 0;
+    n.s2 = 0;
+    const int64_t my = idx;
+    if (!busy && idx < end) {
+      has = Chain::phase_a(rng, n, A) != 0;
+      if (!has) dump_neutron(A, my, n, 0);
+      ++idx;
+    }
+    const int s = q_append(q, has);
+    if (has) {
+      q_store(q, s, n, my);
+      q.s0[s] = rng.s0;
+      q.s1[s] = rng.s1;
+      q.owner[s] = lt;
+      busy = true;
+    }
+    int n_new;
+    bool can_a;
+    round_totals(q, par, has, !busy && idx < end, n_new, can_a);
+    cnt += n_new;  // <= MCV_BLOCK: one queued neutron per busy lane
+    if (cnt >= MCV_BLOCK / 2 || (!can_a && cnt > 0)) {
+      const int take = can_a ? (cnt & ~63) : cnt;
+      if (lt < take) {
+        Neutron b;
+        const int slot = cnt - take + lt;
+        const int64_t bidx = q_load(q, slot, b);
+        Rng r;
+        r.s0 = q.s0[slot];
+        r.s1 = q.s1[slot];
+        const int alive = (int)Chain::phase_b(r, b, A);
+        dump_neutron(A, bidx, b, alive);
+        const int o = q.owner[slot];
+        q.r0[o] = r.s0;
+        q.r1[o] = r.s1;
+        q.returned[o] = 1;
+      }
+      cnt -= take;
+      __syncthreads();
+      if (q.returned[lt]) {
+        rng.s0 = q.r0[lt];
+        rng.s1 = q.r1[lt];
+        q.returned[lt] = 0;
+        busy = false;
+      }
+      if (lt == 0) q.count = cnt;
+      __syncthreads();
+    } else if (!can_a) {
+      break;  // nothing queued and no lane has work
+    }
+  }
+  count_processed(A.counter, mine);
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ batched scans
@@ -206,4 +341,17 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \
     while (mcv::next_item(B, A, tid)) mcv::run_block_compacted<CHAIN>(A, tid, q);             \
+  }
+
+#define MCV_DEFINE_KERNELS_COMPACTED_RNG(NAME, CHAIN)                                          \
+  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME(const mcv::LaunchArgs A) {     \
+    __shared__ mcv::QueueRngLDS q;                                                            \
+    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
+    mcv::run_block_compacted_rng<CHAIN>(A, tid, q);                                           \
+  }                                                                                           \
+  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME##_batch(const mcv::BatchArgs B) { \
+    __shared__ mcv::QueueRngLDS q;                                                            \
+    mcv::LaunchArgs A;                                                                        \
+    int64_t tid;                                                                              \
+    while (mcv::next_item(B, A, tid)) mcv::run_block_compacted_rng<CHAIN>(A, tid, q);         \
   }

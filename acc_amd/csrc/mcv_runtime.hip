@@ -381,6 +381,7 @@ __global__ void mcv_zero_f64(double* p, int64_t n) {
 
 struct mcv_instr {
   int ncomp = 0;
+  int block = 256;  // workgroup size of the code object's kernels
   std::vector<int> types;
   std::vector<int> nparams;  // reference-order params per component (settable)
   mcv::LaunchArgs args{};
@@ -402,6 +403,23 @@ struct mcv_instr {
 };
 
 namespace {
+
+// Workgroup size a chain code object was built for (its `mcv_block_size`
+// global; 256 for code objects without one).
+int module_block(hipModule_t mod, int* block) {
+  *block = 256;
+  hipDeviceptr_t p = nullptr;
+  size_t sz = 0;
+  if (hipModuleGetGlobal(&p, &sz, mod, "mcv_block_size") != hipSuccess) {
+    (void)hipGetLastError();
+    return MCV_OK;
+  }
+  if (sz != sizeof(int)) return fail(MCV_ERR_MODULE, "mcv_block_size has size %zu", sz);
+  HIPCHECK(hipMemcpyDtoH(block, p, sizeof(int)));
+  if (*block < 64 || *block > 1024 || *block % 64)
+    return fail(MCV_ERR_MODULE, "code object built for an unsupported workgroup size %d", *block);
+  return MCV_OK;
+}
 
 void release(mcv_instr* h) {
   if (!h) return;
@@ -448,7 +466,7 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   e = hipModuleGetFunction(&h->fn, h->mod, kernel_name);
   if (e != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s: %s", kernel_name, code_object, hipGetErrorString(e));
-  return MCV_OK;
+  return module_block(h->mod, &h->block);
 }
 
 void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks, int64_t& nthreads, int64_t& npt) {
@@ -538,7 +556,7 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   A.npt = npt;
   A.tid_base = 0;
   A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
-  const unsigned bs = 256;
+  const unsigned bs = (unsigned)h->block;
   const int64_t grid = (nthreads + bs - 1) / bs;
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, 0, h->stream, kargs, nullptr));
@@ -621,7 +639,7 @@ void mcv_instr_destroy(mcv_instr* h) { release(h); }
 
 // ------------------------------------------------------------------ batched scans
 struct mcv_batch {
-  int ninst = 0, ncomp = 0;
+  int ninst = 0, ncomp = 0, block = 256;
   mcv::LaunchArgs args{};
   int64_t pstride = 0, hstride = 0;
   std::vector<int64_t> nbins;
@@ -710,7 +728,7 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   std::string bn = std::string(kernel_name) + "_batch";
   if (hipModuleGetFunction(&b->fn_batch, b->mod, bn.c_str()) != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s", bn.c_str(), code_object);
-  return MCV_OK;
+  return module_block(b->mod, &b->block);
 }
 
 int ensure_rng(mcv_batch* b, int64_t n) {
@@ -757,7 +775,8 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.ninst = b->ninst;
   const int64_t g = std::min<int64_t>(grid > 0 ? grid : 1024, std::max<int64_t>(total_items, 1));
   void* kargs[] = {&B};
-  HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, 256, 1, 1, 0, b->stream, kargs, nullptr));
+  HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, (unsigned)b->block, 1, 1, 0, b->stream, kargs,
+                                 nullptr));
   return MCV_OK;
 }
 
@@ -789,7 +808,7 @@ int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t 
   for (int k = 0; k < n; ++k) {
     int64_t nb, nt, npt;
     partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
-    meta[k + 1] = meta[k] + (nt + 255) / 256;
+    meta[k + 1] = meta[k] + (nt + b->block - 1) / b->block;
     meta[n + 1 + k] = (int64_t)ncounts[k];
     meta[2 * n + 1 + k] = nt;
     meta[3 * n + 1 + k] = npt;
@@ -889,7 +908,8 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.tid_base = 0;
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
-    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + 255) / 256), 1, 1, 256, 1, 1, 0, b->stream, kargs,
+    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + b->block - 1) / b->block), 1, 1, (unsigned)b->block, 1,
+                                   1, 0, b->stream, kargs,
                                    nullptr));
     HIPCHECK(hipEventRecord(b->ev[1], b->stream));
     HIPCHECK(hipStreamSynchronize(b->stream));

@@ -56,26 +56,33 @@ def chain_body(components):
 
 
 def split_index(components):
-    """Index k splitting the chain into phase A = [0..k] and phase B = [k+1..]
-    for the compacted kernel, or None.  Phase B must draw no random numbers
-    (each stream's draws then stay in the reference's order) and must hold a
-    component that is not a monitor (otherwise there is nothing to compact).
-    Phase A ends at the first non-monitor component after the last drawing
-    one: the first point where neutrons die in numbers (a guide entrance)."""
+    """Index of the last component of phase A (the per-stream part) of the
+    compacted kernel, or None.  Phase A ends at the first component after the
+    first drawing one (the source) that is neither a monitor nor an Arm: the
+    first point where neutrons die in numbers (a guide entrance, a sample).
+    Compaction needs a phase B holding a component that is not a monitor
+    (otherwise there is nothing to run densely)."""
     from .components.optics.arm import Arm
     stoch = [i for i, c in enumerate(components) if c.propagate.stochastic]
-    last = stoch[-1] if stoch else -1
-    k = last
-    for j in range(last + 1, len(components)):
+    if not stoch:
+        return None
+    k = None
+    for j in range(stoch[0] + 1, len(components)):
         c = components[j]
         if not c.propagate.monitor and not isinstance(c, Arm):
             k = j
             break
-    if k < 0 or k >= len(components) - 1:
+    if k is None or k >= len(components) - 1:
         return None
     if all(c.propagate.monitor or isinstance(c, Arm) for c in components[k + 1:]):
         return None
     return k
+
+
+def phase_b_draws(components, k):
+    """Whether phase B (components after k) draws random numbers: its
+    neutrons then carry their stream's state through the queue."""
+    return any(c.propagate.stochastic for c in components[k + 1:])
 
 
 def _phase(components, lo, hi, first_transform):
@@ -90,8 +97,18 @@ def _phase(components, lo, hi, first_transform):
     return "\n  ".join(lines)
 
 
-def generate_source(instrument, compact=True):
-    """(source text, structural description) of the fused kernel."""
+def generate_source(instrument, compact=True, block=None):
+    """(source text, structural description) of the fused kernel, for
+    workgroups of `block` threads (default config.kernel_block)."""
+    from . import config
+    block = int(block or config.kernel_block)
+    if block % 64 or not 64 <= block <= 1024:
+        raise ValueError("workgroup size must be a multiple of 64 in [64, 1024], got %d" % block)
+    src, structure = _generate(instrument, compact)
+    return src.replace('#include "mcv_kernel.h"', '#define MCV_BLOCK %d\n#include "mcv_kernel.h"' % block, 1), structure
+
+
+def _generate(instrument, compact):
     comps = instrument.components
     if not comps:
         raise ValueError("empty instrument")
@@ -107,6 +124,10 @@ def generate_source(instrument, compact=True):
     if k is not None:
         a = _phase(comps, 0, k + 1, False)
         b = _phase(comps, k + 1, len(comps), True)
+        draws = phase_b_draws(comps, k)
+        b_sig = "mcv::Rng& rng, mcv::Neutron& n" if draws else "mcv::Neutron& n"
+        b_void = "(void)rng; (void)H;" if draws else "(void)H;"
+        macro = "MCV_DEFINE_KERNELS_COMPACTED_RNG" if draws else "MCV_DEFINE_KERNELS_COMPACTED"
         src = f"""// fused instrument kernel, compacted after component {k}: {structure}
 #include "mcv_kernel.h"
 
@@ -120,18 +141,19 @@ struct Chain {{
   {a}
   }}
 
-  __device__ static __forceinline__ int phase_b(mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int phase_b({b_sig}, const mcv::LaunchArgs& A) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  (void)H;
+  {b_void}
   {b}
   }}
 }};
 }}  // namespace
 
-MCV_DEFINE_KERNELS_COMPACTED(KERNEL_NAME, Chain)
+{macro}(KERNEL_NAME, Chain)
 """
-        return src, structure + f" [compacted after {k}]"
+        tag = " [compacted after %d%s]" % (k, ", streams carried" if draws else "")
+        return src, structure + tag
     body = "\n  ".join(chain_body(comps))
     src = f"""// fused instrument kernel: {structure}
 #include "mcv_kernel.h"
@@ -187,7 +209,7 @@ def compile_source(src, verbose=False):
     return out, name
 
 
-def fuse(instrument, verbose=False, compact=True):
+def fuse(instrument, verbose=False, compact=True, block=None):
     """Compile (or fetch from the cache) the fused kernel of an instrument."""
-    src, _ = generate_source(instrument, compact=compact)
+    src, _ = generate_source(instrument, compact=compact, block=block)
     return compile_source(src, verbose=verbose)

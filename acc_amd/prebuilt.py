@@ -135,7 +135,13 @@ def instruments():
         "csg-difference": lambda: _csg_chain("difference_example1.xml", (90, 0, 0)),
         "iqe-normalizer": _iqe_normalizer,
         "scan": lambda: configs.scan_instance(60.0, 30.0),
+        "scan-plate": lambda: configs.scan_instance(60.0, 30.0, sample="plate"),
     }
+
+
+# chains also built for the other workgroup sizes the autotuner tries
+TUNED = ("c2", "c2s", "scan")
+TUNED_BLOCKS = (128, 512)
 
 
 def build_all(verbose=False, jobs=None):
@@ -143,8 +149,21 @@ def build_all(verbose=False, jobs=None):
     from concurrent.futures import ThreadPoolExecutor
     from .fuser import generate_source, compile_source
     srcs = {name: generate_source(make())[0] for name, make in instruments().items()}
+    for name in TUNED:
+        for block in TUNED_BLOCKS:
+            srcs["%s@%d" % (name, block)] = generate_source(instruments()[name](), block=block)[0]
     unique = sorted(set(srcs.values()))
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:
         built = dict(zip(unique, ex.map(lambda s: compile_source(s, verbose=verbose), unique)))
     return {name: built[s] for name, s in srcs.items()}
+
+
+def prune(keep):
+    """Delete cached chain kernels other than `keep` ({name: (path, kernel)})."""
+    import glob
+    from .fuser import CACHE_DIR
+    names = {k for _, k in keep.values()}
+    for path in glob.glob(os.path.join(CACHE_DIR, "mcv_fused_*")):
+        if os.path.basename(path).split(".")[0] not in names:
+            os.remove(path)

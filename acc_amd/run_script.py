@@ -67,17 +67,20 @@ def compile(script, compiled_script=None, **kwds):
     return path
 
 
-def get_handle(instrument):
-    """The device handle of an instrument (built on first use, then reused)."""
-    h = getattr(instrument, "_mcv_handle", None)
+def get_handle(instrument, block=None):
+    """The device handle of an instrument for workgroups of `block` threads
+    (default config.kernel_block); built on first use, then reused."""
+    block = int(block or config.kernel_block)
+    handles = instrument.__dict__.setdefault("_mcv_handles", {})
+    h = handles.get(block)
     if h is not None:
         return h
     from .fuser import fuse
-    code_object, kernel_name = fuse(instrument)
+    code_object, kernel_name = fuse(instrument, block=block)
     offsets, rotmats = calcTransformations(instrument)
     descs = [c.mcv_desc() for c in instrument.components]
     h = Handle(descs, offsets, rotmats, code_object, kernel_name)
-    instrument._mcv_handle = h
+    handles[block] = h
     return h
 
 

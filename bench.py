@@ -87,6 +87,48 @@ def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0):
                 streams, nthreads, done, dt)}
 
 
+def batch_scan(ninst=64, reps=3):
+    """The 64-instance scan (Ei 20-200 meV log-uniform, sample rotation
+    0-180 deg, counts log-uniform 1e5-1e7, default_rng(0)): one persistent
+    hipGraph launch vs the same instances launched back to back, and vs a
+    loop of run_instrument calls (the one-run-per-instrument API)."""
+    from acc_amd import configs
+    from acc_amd.batch import BatchedScan
+    from acc_amd.run_script import run_instrument
+    ins, n = configs.scan(ninst)
+    total = int(n.sum())
+    scan = BatchedScan(ins)
+    try:
+        scan.run(n, seed=0)
+        scan.run_sequential(n, seed=0)
+        b = [scan.run(n, seed=0)["ms_kernel"] for _ in range(reps)]
+        t = time.perf_counter()
+        for _ in range(reps):
+            scan.run(n, seed=0)
+        b_wall = (time.perf_counter() - t) / reps * 1e3
+        s = [scan.run_sequential(n, seed=0)["ms_kernel"] for _ in range(reps)]
+        t = time.perf_counter()
+        for _ in range(reps):
+            scan.run_sequential(n, seed=0)
+        s_wall = (time.perf_counter() - t) / reps * 1e3
+    finally:
+        scan.close()
+    singles = [configs.scan_instance(float(e), float(r), "one%d" % k)
+               for k, (e, r) in enumerate(zip(*configs.scan_plan(ninst)[:2]))]
+    for instr, c in zip(singles, n):  # builds the handles (module loads) outside the timing
+        run_instrument(instr, int(min(c, 1000)), seed=0)
+    t = time.perf_counter()
+    for instr, c in zip(singles, n):
+        run_instrument(instr, int(c), seed=0)
+    api_wall = (time.perf_counter() - t) * 1e3
+    bk = sum(b) / len(b)
+    return {"instances": ninst, "neutrons": total, "ms_batched_graph_kernel": bk, "ms_batched_graph_wall": b_wall,
+            "ms_back_to_back_kernels": sum(s) / len(s), "ms_back_to_back_wall": s_wall,
+            "ms_run_instrument_loop_wall": api_wall,
+            "speedup_vs_back_to_back": s_wall / b_wall, "speedup_vs_run_instrument_loop": api_wall / b_wall,
+            "neutrons_per_s_batched": total / (b_wall * 1e-3)}
+
+
 def traffic_from_profiles(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
@@ -104,6 +146,7 @@ def main():
     ap.add_argument("--ncount", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--falg-neutrons", type=int, default=1000000)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
@@ -150,6 +193,8 @@ def main():
                    "semantics": "dead neutrons (p<=0) dropped"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if not a.no_batch:
+        line["batch_scan"] = batch_scan()
     print(json.dumps(line))
     return 0
 

@@ -16,10 +16,14 @@ def test_every_packaged_chain_compiles(name):
 
 def test_split_points():
     I = prebuilt.instruments()
-    assert fuser.split_index(I["c2"]().components) == 1          # after the straight guide
+    c2 = I["c2"]().components
+    assert fuser.split_index(c2) == 1 and not fuser.phase_b_draws(c2, 1)  # after the straight guide
     assert fuser.split_index(I["c2-straight"]().components) is None  # only monitors after the guide
-    assert fuser.split_index(I["c1"]().components) is None        # sample draws: nothing deterministic left
-    assert fuser.split_index(I["c2s"]().components) is None       # sample draws after the guides
+    assert fuser.split_index(I["c1"]().components) is None        # only monitors after the sample
+    c2s = I["c2s"]().components
+    assert fuser.split_index(c2s) == 1 and fuser.phase_b_draws(c2s, 1)  # the sample draws in phase B
+    src = fuser.generate_source(I["c2s"]())[0]
+    assert "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src and "#define MCV_BLOCK 256" in src
 
 
 def test_structure_only_enters_the_source():

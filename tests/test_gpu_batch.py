@@ -100,4 +100,4 @@ def test_batch_new_counts_recapture(scan_and_expected):
 def test_batch_rejects_different_structure():
     from acc_amd.prebuilt import instruments
     with pytest.raises(ValueError):
-        BatchedScan([configs.scan_instance(60.0, 0.0), instruments()["c2s"]()])
+        BatchedScan([configs.scan_instance(60.0, 0.0), instruments()["c2"]()])

@@ -1,0 +1,53 @@
+"""Launch shape: the workgroup size a chain is compiled for changes nothing in
+the results (each stream's neutrons are computed the same way), and the
+autotuner returns a usable choice."""
+import numpy as np
+import pytest
+
+from acc_amd import autotune, configs, prebuilt
+from acc_amd._abi import MCV_FLAG_GATE_DEAD
+from acc_amd.run_script import get_handle
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(instr, block, N, ntot):
+    h = get_handle(instr, block=block)
+    h.set_dump(0, N)
+    h.clear()
+    st = h.run(N, 3, ntot, 512, MCV_FLAG_GATE_DEAD)
+    assert st["processed"] == N
+    states, alive = h.read_dump()
+    hists = {i: h.read_monitor(i) for i in range(len(instr.components)) if h.nbins[i] > 0}
+    return states, alive, hists
+
+
+@pytest.mark.parametrize("name", ["c2", "c2s"])
+@pytest.mark.parametrize("block", [128, 512])
+def test_block_size_invariance(name, block):
+    print(f"\n[launch] {name} block {block}", flush=True)
+    instr = prebuilt.instruments()[name]()
+    N, ntot = 300000, 20000
+    s0, a0, h0 = _run(instr, 256, N, ntot)
+    s1, a1, h1 = _run(instr, block, N, ntot)
+    np.testing.assert_array_equal(a1, a0)
+    np.testing.assert_array_equal(s1, s0)
+    for i in h0:
+        np.testing.assert_array_equal(h1[i][0], h0[i][0])
+        np.testing.assert_allclose(h1[i][1:], h0[i][1:], rtol=1e-12, atol=1e-300)
+
+
+def test_autotune_scan_small():
+    print("\n[launch] autotune scan", flush=True)
+    Ei, rot = [30.0, 90.0, 150.0], [0.0, 60.0, 120.0]
+    ins = [configs.scan_instance(e, r, "t%d" % k) for k, (e, r) in enumerate(zip(Ei, rot))]
+    n = [20000, 150000, 3000]
+    best = autotune.tune_scan(ins, n, blocks=(128, 256), grids=(0, 256), use_cache=False)
+    assert best["block"] in (128, 256) and best["grid"] in (0, 256)
+    assert len(best["trials"]) == 4 and all(t["ms"] > 0 for t in best["trials"])
+    scan, kw = autotune.tuned_scan(ins, n, blocks=(128, 256), grids=(0, 256))
+    try:
+        st = scan.run(n, seed=1, **kw)
+        assert st["processed"] == sum(n)
+    finally:
+        scan.close()
