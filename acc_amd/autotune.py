@@ -7,6 +7,9 @@ it (acc/components/sources/SourceBase.py:36-52).  Here the shape is measured:
  * block: workgroup size the chain is compiled for (multiples of 64).  Each
    stream's work does not depend on it, so results are identical for every
    block; it changes occupancy and the width of the compaction queue.
+ * waves_per_eu: occupancy target the register allocator must meet (None =
+   the compiler's choice).  Chains whose natural allocation leaves 2 waves per
+   SIMD can run faster at 3-4 with some spills, or slower: it is measured.
  * grid: number of persistent workgroups of a batched scan (0 = default).
  * ntotalthreads (only when `neutrons_per_thread=True`): number of random
    streams, i.e. neutrons per thread.  This changes which stream draws which
@@ -25,7 +28,8 @@ import threading
 from . import config
 
 DEFAULT_BLOCKS = (128, 256, 512)
-DEFAULT_GRIDS = (0, 512, 2048)
+DEFAULT_WAVES = (None, 3, 4)
+DEFAULT_GRIDS = (0,)
 DEFAULT_NTOTALTHREADS = (250000, 500000, 1000000, 2000000)
 _lock = threading.Lock()
 
@@ -57,7 +61,7 @@ def _store(key, value):
 
 def _key(kind, instrument, workload):
     from .fuser import generate_source
-    src = generate_source(instrument, block=64)[0]  # block-independent text
+    src = generate_source(instrument, block=64, waves_per_eu=0)[0]  # text without the launch shape
     h = hashlib.sha1(src.encode())
     h.update(json.dumps(workload, sort_keys=True).encode())
     return "%s:%s" % (kind, h.hexdigest()[:16])
@@ -67,13 +71,14 @@ def _best(trials):
     return min(trials, key=lambda t: t["ms"])
 
 
-def tune_instrument(instrument, ncount, seed=0, blocks=DEFAULT_BLOCKS, neutrons_per_thread=False,
-                    ntotalthreads=DEFAULT_NTOTALTHREADS, reps=2, use_cache=True):
-    """Best {block, ntotalthreads, ms} for `ncount` neutrons of `instrument`."""
+def tune_instrument(instrument, ncount, seed=0, blocks=DEFAULT_BLOCKS, waves=DEFAULT_WAVES,
+                    neutrons_per_thread=False, ntotalthreads=DEFAULT_NTOTALTHREADS, reps=2, use_cache=True):
+    """Best {block, waves_per_eu, ntotalthreads, ms} for `ncount` neutrons of
+    `instrument` (the histograms it leaves behind are those of the last trial)."""
     from .run_script import get_handle
     from ._abi import MCV_FLAG_GATE_DEAD
     nts = tuple(ntotalthreads) if neutrons_per_thread else (config.ntotalthreads,)
-    workload = {"ncount": int(ncount), "blocks": list(blocks), "nts": list(nts)}
+    workload = {"ncount": int(ncount), "blocks": list(blocks), "waves": list(waves), "nts": list(nts)}
     key = _key("instrument", instrument, workload)
     if use_cache:
         hit = _load().get(key)
@@ -81,25 +86,26 @@ def tune_instrument(instrument, ncount, seed=0, blocks=DEFAULT_BLOCKS, neutrons_
             return hit
     trials = []
     for block in blocks:
-        h = get_handle(instrument, block=block)
-        for nt in nts:
-            h.clear()
-            h.run(int(ncount), seed, nt, config.threads_per_block, MCV_FLAG_GATE_DEAD)
-            ms = min(h.run(int(ncount), seed, nt, config.threads_per_block, MCV_FLAG_GATE_DEAD)["ms_kernel"]
-                     for _ in range(reps))
-            trials.append({"block": int(block), "ntotalthreads": int(nt), "ms": float(ms)})
+        for w in waves:
+            h = get_handle(instrument, block=block, waves_per_eu=w)
+            for nt in nts:
+                h.clear()
+                h.run(int(ncount), seed, nt, config.threads_per_block, MCV_FLAG_GATE_DEAD)
+                ms = min(h.run(int(ncount), seed, nt, config.threads_per_block, MCV_FLAG_GATE_DEAD)["ms_kernel"]
+                         for _ in range(reps))
+                trials.append({"block": int(block), "waves_per_eu": w, "ntotalthreads": int(nt), "ms": float(ms)})
     best = dict(_best(trials), trials=trials)
     _store(key, best)
     return best
 
 
-def tune_scan(instruments, ncounts, seed=0, blocks=DEFAULT_BLOCKS, grids=DEFAULT_GRIDS,
+def tune_scan(instruments, ncounts, seed=0, blocks=DEFAULT_BLOCKS, waves=DEFAULT_WAVES, grids=DEFAULT_GRIDS,
               neutrons_per_thread=False, ntotalthreads=DEFAULT_NTOTALTHREADS, reps=2, use_cache=True):
-    """Best {block, grid, ntotalthreads, ms} for a batched scan."""
+    """Best {block, waves_per_eu, grid, ntotalthreads, ms} for a batched scan."""
     from .batch import BatchedScan
     nts = tuple(ntotalthreads) if neutrons_per_thread else (config.ntotalthreads,)
-    workload = {"ncounts": [int(n) for n in ncounts], "blocks": list(blocks), "grids": list(grids),
-                "nts": list(nts), "ninst": len(instruments)}
+    workload = {"ncounts": [int(n) for n in ncounts], "blocks": list(blocks), "waves": list(waves),
+                "grids": list(grids), "nts": list(nts), "ninst": len(instruments)}
     key = _key("scan", instruments[0], workload)
     if use_cache:
         hit = _load().get(key)
@@ -107,25 +113,33 @@ def tune_scan(instruments, ncounts, seed=0, blocks=DEFAULT_BLOCKS, grids=DEFAULT
             return hit
     trials = []
     for block in blocks:
-        scan = BatchedScan(instruments, block=block)
-        try:
-            for grid in grids:
-                for nt in nts:
-                    scan.run(ncounts, seed=seed, ntotalthreads=nt, grid=grid)
-                    ms = min(scan.run(ncounts, seed=seed, ntotalthreads=nt, grid=grid)["ms_kernel"]
-                             for _ in range(reps))
-                    trials.append({"block": int(block), "grid": int(grid), "ntotalthreads": int(nt),
-                                   "ms": float(ms)})
-        finally:
-            scan.close()
+        for w in waves:
+            scan = BatchedScan(instruments, block=block, waves_per_eu=w)
+            try:
+                for grid in grids:
+                    for nt in nts:
+                        scan.run(ncounts, seed=seed, ntotalthreads=nt, grid=grid)
+                        ms = min(scan.run(ncounts, seed=seed, ntotalthreads=nt, grid=grid)["ms_kernel"]
+                                 for _ in range(reps))
+                        trials.append({"block": int(block), "waves_per_eu": w, "grid": int(grid),
+                                       "ntotalthreads": int(nt), "ms": float(ms)})
+            finally:
+                scan.close()
     best = dict(_best(trials), trials=trials)
     _store(key, best)
     return best
 
 
 def tuned_scan(instruments, ncounts, **kw):
-    """(BatchedScan built for the tuned block, keyword arguments for its run())."""
+    """(BatchedScan built for the tuned launch shape, keyword arguments for its run())."""
     from .batch import BatchedScan
     best = tune_scan(instruments, ncounts, **kw)
-    scan = BatchedScan(instruments, block=best["block"])
+    scan = BatchedScan(instruments, block=best["block"], waves_per_eu=best["waves_per_eu"])
     return scan, {"grid": best["grid"], "ntotalthreads": best["ntotalthreads"]}
+
+
+def tuned_handle(instrument, ncount, **kw):
+    """(device handle built for the tuned launch shape, the tuning record)."""
+    from .run_script import get_handle
+    best = tune_instrument(instrument, ncount, **kw)
+    return get_handle(instrument, block=best["block"], waves_per_eu=best["waves_per_eu"]), best

@@ -26,15 +26,18 @@ from .instrument import calcTransformations
 class BatchedScan:
     """Instances of one chain structure, run together on the GPU."""
 
-    def __init__(self, instruments, compact=True, block=None):
+    def __init__(self, instruments, compact=True, block=None, waves_per_eu=None):
         from .fuser import generate_source, compile_source
         if not instruments:
             raise ValueError("empty scan")
-        srcs = [generate_source(instr, compact=compact, block=block)[0] for instr in instruments]
+        srcs = [generate_source(instr, compact=compact, block=block, waves_per_eu=waves_per_eu)[0]
+                for instr in instruments]
         for k, s in enumerate(srcs):
             if s != srcs[0]:
                 raise ValueError("instance %d has a different chain structure than instance 0" % k)
         self.code_object, self.kernel_name = compile_source(srcs[0])
+        self.block = int(block or config.kernel_block)
+        self.waves_per_eu = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
         self.instruments = list(instruments)
         descs, offs, rots = [], [], []
         for instr in self.instruments:

@@ -14,6 +14,8 @@ threads_per_block = 512
 # workgroup size of the fused kernels (results do not depend on it; the
 # reference's threads_per_block only shapes the stream partition)
 kernel_block = 256
+# occupancy target (waves per SIMD) of the fused kernels; None = compiler's choice
+kernel_waves_per_eu = None
 floattype = "float64"
 rng_seed = 1
 gate_dead_neutrons = True

@@ -33,6 +33,14 @@
 
 extern "C" __device__ int mcv_block_size = MCV_BLOCK;
 
+// Optional occupancy target (waves per SIMD) the register allocator must meet;
+// above the natural occupancy of a chain it trades spills for latency hiding.
+#ifdef MCV_WAVES_PER_EU
+#define MCV_KERNEL_ATTR __launch_bounds__(MCV_BLOCK) __attribute__((amdgpu_waves_per_eu(MCV_WAVES_PER_EU)))
+#else
+#define MCV_KERNEL_ATTR __launch_bounds__(MCV_BLOCK)
+#endif
+
 namespace mcv {
 
 __device__ __forceinline__ void dump_neutron(const LaunchArgs& A, int64_t idx, const Neutron& n, int alive) {
@@ -320,23 +328,23 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 }  // namespace mcv
 
 #define MCV_DEFINE_KERNELS_PLAIN(NAME, CHAIN)                                                  \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME(const mcv::LaunchArgs A) {     \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
     mcv::run_lane<CHAIN>(A, tid);                                                             \
   }                                                                                           \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME##_batch(const mcv::BatchArgs B) { \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \
     while (mcv::next_item(B, A, tid)) mcv::run_lane<CHAIN>(A, tid);                           \
   }
 
 #define MCV_DEFINE_KERNELS_COMPACTED(NAME, CHAIN)                                              \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME(const mcv::LaunchArgs A) {     \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     __shared__ mcv::QueueLDS q;                                                               \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
     mcv::run_block_compacted<CHAIN>(A, tid, q);                                               \
   }                                                                                           \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME##_batch(const mcv::BatchArgs B) { \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     __shared__ mcv::QueueLDS q;                                                               \
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \
@@ -344,12 +352,12 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   }
 
 #define MCV_DEFINE_KERNELS_COMPACTED_RNG(NAME, CHAIN)                                          \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME(const mcv::LaunchArgs A) {     \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     __shared__ mcv::QueueRngLDS q;                                                            \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
     mcv::run_block_compacted_rng<CHAIN>(A, tid, q);                                           \
   }                                                                                           \
-  extern "C" __global__ void __launch_bounds__(MCV_BLOCK) NAME##_batch(const mcv::BatchArgs B) { \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     __shared__ mcv::QueueRngLDS q;                                                            \
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \

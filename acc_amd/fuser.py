@@ -97,15 +97,23 @@ def _phase(components, lo, hi, first_transform):
     return "\n  ".join(lines)
 
 
-def generate_source(instrument, compact=True, block=None):
+def generate_source(instrument, compact=True, block=None, waves_per_eu=None):
     """(source text, structural description) of the fused kernel, for
-    workgroups of `block` threads (default config.kernel_block)."""
+    workgroups of `block` threads (default config.kernel_block) and, if
+    given, an occupancy target of `waves_per_eu` waves per SIMD (default
+    config.kernel_waves_per_eu; None leaves it to the compiler)."""
     from . import config
     block = int(block or config.kernel_block)
     if block % 64 or not 64 <= block <= 1024:
         raise ValueError("workgroup size must be a multiple of 64 in [64, 1024], got %d" % block)
+    wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
+    defs = "#define MCV_BLOCK %d\n" % block
+    if wpe:
+        if not 1 <= int(wpe) <= 8:
+            raise ValueError("waves per SIMD must be in [1, 8], got %r" % wpe)
+        defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
     src, structure = _generate(instrument, compact)
-    return src.replace('#include "mcv_kernel.h"', '#define MCV_BLOCK %d\n#include "mcv_kernel.h"' % block, 1), structure
+    return src.replace('#include "mcv_kernel.h"', defs + '#include "mcv_kernel.h"', 1), structure
 
 
 def _generate(instrument, compact):
@@ -209,7 +217,7 @@ def compile_source(src, verbose=False):
     return out, name
 
 
-def fuse(instrument, verbose=False, compact=True, block=None):
+def fuse(instrument, verbose=False, compact=True, block=None, waves_per_eu=None):
     """Compile (or fetch from the cache) the fused kernel of an instrument."""
-    src, _ = generate_source(instrument, compact=compact, block=block)
+    src, _ = generate_source(instrument, compact=compact, block=block, waves_per_eu=waves_per_eu)
     return compile_source(src, verbose=verbose)

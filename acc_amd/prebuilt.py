@@ -139,9 +139,8 @@ def instruments():
     }
 
 
-# chains also built for the other workgroup sizes the autotuner tries
-TUNED = ("c2", "c2s", "scan")
-TUNED_BLOCKS = (128, 512)
+# chains also built for every launch shape the autotuner tries
+TUNED = ("c2", "c2s")
 
 
 def build_all(verbose=False, jobs=None):
@@ -149,9 +148,12 @@ def build_all(verbose=False, jobs=None):
     from concurrent.futures import ThreadPoolExecutor
     from .fuser import generate_source, compile_source
     srcs = {name: generate_source(make())[0] for name, make in instruments().items()}
+    from .autotune import DEFAULT_BLOCKS, DEFAULT_WAVES
     for name in TUNED:
-        for block in TUNED_BLOCKS:
-            srcs["%s@%d" % (name, block)] = generate_source(instruments()[name](), block=block)[0]
+        for block in DEFAULT_BLOCKS:
+            for w in DEFAULT_WAVES:
+                srcs["%s@%d/%s" % (name, block, w)] = generate_source(instruments()[name](), block=block,
+                                                                      waves_per_eu=w)[0]
     unique = sorted(set(srcs.values()))
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:

@@ -67,20 +67,22 @@ def compile(script, compiled_script=None, **kwds):
     return path
 
 
-def get_handle(instrument, block=None):
+def get_handle(instrument, block=None, waves_per_eu=None):
     """The device handle of an instrument for workgroups of `block` threads
-    (default config.kernel_block); built on first use, then reused."""
+    (default config.kernel_block) and an optional occupancy target; built on
+    first use, then reused."""
     block = int(block or config.kernel_block)
+    wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
     handles = instrument.__dict__.setdefault("_mcv_handles", {})
-    h = handles.get(block)
+    h = handles.get((block, wpe))
     if h is not None:
         return h
     from .fuser import fuse
-    code_object, kernel_name = fuse(instrument, block=block)
+    code_object, kernel_name = fuse(instrument, block=block, waves_per_eu=wpe)
     offsets, rotmats = calcTransformations(instrument)
     descs = [c.mcv_desc() for c in instrument.components]
     h = Handle(descs, offsets, rotmats, code_object, kernel_name)
-    handles[block] = h
+    handles[(block, wpe)] = h
     return h
 
 

@@ -95,16 +95,18 @@ def batch_scan(ninst=64, reps=3):
     from acc_amd import configs
     from acc_amd.batch import BatchedScan
     from acc_amd.run_script import run_instrument
+    from acc_amd.autotune import tuned_scan
     ins, n = configs.scan(ninst)
     total = int(n.sum())
-    scan = BatchedScan(ins)
+    scan, kw = tuned_scan(ins, n, use_cache=False)
+    tuned = {"block": scan.block, "waves_per_eu": scan.waves_per_eu}
     try:
-        scan.run(n, seed=0)
+        scan.run(n, seed=0, **kw)
         scan.run_sequential(n, seed=0)
-        b = [scan.run(n, seed=0)["ms_kernel"] for _ in range(reps)]
+        b = [scan.run(n, seed=0, **kw)["ms_kernel"] for _ in range(reps)]
         t = time.perf_counter()
         for _ in range(reps):
-            scan.run(n, seed=0)
+            scan.run(n, seed=0, **kw)
         b_wall = (time.perf_counter() - t) / reps * 1e3
         s = [scan.run_sequential(n, seed=0)["ms_kernel"] for _ in range(reps)]
         t = time.perf_counter()
@@ -113,20 +115,27 @@ def batch_scan(ninst=64, reps=3):
         s_wall = (time.perf_counter() - t) / reps * 1e3
     finally:
         scan.close()
+    from acc_amd import config
     singles = [configs.scan_instance(float(e), float(r), "one%d" % k)
                for k, (e, r) in enumerate(zip(*configs.scan_plan(ninst)[:2]))]
-    for instr, c in zip(singles, n):  # builds the handles (module loads) outside the timing
-        run_instrument(instr, int(min(c, 1000)), seed=0)
-    t = time.perf_counter()
-    for instr, c in zip(singles, n):
-        run_instrument(instr, int(c), seed=0)
-    api_wall = (time.perf_counter() - t) * 1e3
+    saved = config.kernel_block, config.kernel_waves_per_eu
+    config.kernel_block, config.kernel_waves_per_eu = tuned["block"], tuned["waves_per_eu"]
+    try:
+        for instr, c in zip(singles, n):  # builds the handles (module loads) outside the timing
+            run_instrument(instr, int(min(c, 1000)), seed=0)
+        t = time.perf_counter()
+        for instr, c in zip(singles, n):
+            run_instrument(instr, int(c), seed=0)
+        api_wall = (time.perf_counter() - t) * 1e3
+    finally:
+        config.kernel_block, config.kernel_waves_per_eu = saved
     bk = sum(b) / len(b)
     return {"instances": ninst, "neutrons": total, "ms_batched_graph_kernel": bk, "ms_batched_graph_wall": b_wall,
             "ms_back_to_back_kernels": sum(s) / len(s), "ms_back_to_back_wall": s_wall,
             "ms_run_instrument_loop_wall": api_wall,
             "speedup_vs_back_to_back": s_wall / b_wall, "speedup_vs_run_instrument_loop": api_wall / b_wall,
-            "neutrons_per_s_batched": total / (b_wall * 1e-3)}
+            "neutrons_per_s_batched": total / (b_wall * 1e-3), "launch_shape": tuned,
+            "note": "same tuned kernel in all three; back-to-back = one launch per instance, no host work between"}
 
 
 def traffic_from_profiles(workload):
@@ -147,6 +156,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--falg-neutrons", type=int, default=1000000)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
@@ -155,12 +165,19 @@ def main():
 
     from acc_amd import configs
     from acc_amd._abi import MCV_FLAG_GATE_DEAD
+    from acc_amd.autotune import tuned_handle
     from acc_amd.run_script import get_handle
 
     desc, default_n = WORKLOADS[a.workload]
     ncount = int(a.ncount or default_n)
     instr = configs.BUILDERS[a.workload]()
-    h = get_handle(instr)
+    if a.no_tune:
+        h, shape = get_handle(instr), {"block": None, "waves_per_eu": None, "tuned": False}
+    else:
+        h, best = tuned_handle(instr, ncount, use_cache=False)
+        shape = {"block": best["block"], "waves_per_eu": best["waves_per_eu"], "tuned": True,
+                 "trials_ms": {"%d/%s" % (t["block"], t["waves_per_eu"]): round(t["ms"], 2)
+                               for t in best["trials"]}}
     seed = 0
     for _ in range(a.warmup):
         h.clear()
@@ -190,7 +207,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{a.workload}: {desc}", "ncount_per_step": ncount,
                    "partition": "reference default (ntotalthreads 1e6, 512 threads/block)",
-                   "semantics": "dead neutrons (p<=0) dropped"},
+                   "semantics": "dead neutrons (p<=0) dropped", "launch_shape": shape},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if not a.no_batch:

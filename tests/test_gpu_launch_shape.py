@@ -42,10 +42,11 @@ def test_autotune_scan_small():
     Ei, rot = [30.0, 90.0, 150.0], [0.0, 60.0, 120.0]
     ins = [configs.scan_instance(e, r, "t%d" % k) for k, (e, r) in enumerate(zip(Ei, rot))]
     n = [20000, 150000, 3000]
-    best = autotune.tune_scan(ins, n, blocks=(128, 256), grids=(0, 256), use_cache=False)
-    assert best["block"] in (128, 256) and best["grid"] in (0, 256)
-    assert len(best["trials"]) == 4 and all(t["ms"] > 0 for t in best["trials"])
-    scan, kw = autotune.tuned_scan(ins, n, blocks=(128, 256), grids=(0, 256))
+    kw = dict(blocks=(128, 256), waves=(None, 3), grids=(0, 256))
+    best = autotune.tune_scan(ins, n, use_cache=False, **kw)
+    assert best["block"] in (128, 256) and best["grid"] in (0, 256) and best["waves_per_eu"] in (None, 3)
+    assert len(best["trials"]) == 8 and all(t["ms"] > 0 for t in best["trials"])
+    scan, kw = autotune.tuned_scan(ins, n, **kw)
     try:
         st = scan.run(n, seed=1, **kw)
         assert st["processed"] == sum(n)
