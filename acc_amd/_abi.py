@@ -51,7 +51,7 @@ EXPORTED = (
     "mcv_instr_set_params", "mcv_instr_run", "mcv_monitor_read", "mcv_monitor_clear",
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
-    "mcv_batch_clear", "mcv_batch_destroy",
+    "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint",
 )
 
 
@@ -144,6 +144,10 @@ def lib():
     L.mcv_batch_clear.argtypes = [ctypes.c_void_p]
     L.mcv_batch_destroy.restype = None
     L.mcv_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.mcv_instr_adjoint.restype = ctypes.c_int
+    L.mcv_instr_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
+                                    ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, dp,
+                                    ctypes.c_int32, dp, ctypes.c_int64, dp, ctypes.POINTER(McvRunStats)]
     if L.mcv_abi_version() != 1:
         raise ImportError("libmcvacc.so ABI version mismatch")
     _lib = L
@@ -204,6 +208,9 @@ class Handle:
             raise McvError(MCV_ERR_ARG, L.mcv_last_error().decode())
         self.h = h
         self.nbins = [int(d["nbins"]) for d in descs]
+        # histogram layout: (3, nbins) blocks of the components with bins, in chain order
+        self.hoff = list(np.cumsum([0] + [3 * nb for nb in self.nbins])[:-1])
+        self.hist_total = int(3 * sum(self.nbins))
 
     def run(self, ncount, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD):
         st = McvRunStats()
@@ -226,6 +233,21 @@ class Handle:
     def set_dump(self, first, count):
         check(lib().mcv_instr_set_dump(self.h, int(first), int(count)))
         self._dump_count = int(count)
+
+    def adjoint(self, code_object, kernel_name, ncount, seed, g_hist, nscore, ntotalthreads=0,
+                threads_per_block=0, flags=MCV_FLAG_GATE_DEAD, ndense=0):
+        """(grad[nscore], dense gradient or None, run stats) of mcv_instr_adjoint."""
+        g = np.ascontiguousarray(g_hist, dtype=np.float64).reshape(-1)
+        if g.size != self.hist_total:
+            raise ValueError("upstream gradient has %d values, histograms %d" % (g.size, self.hist_total))
+        grad = np.zeros(int(nscore))
+        dense = np.zeros(int(ndense)) if ndense else None
+        st = McvRunStats()
+        check(lib().mcv_instr_adjoint(self.h, code_object.encode(), kernel_name.encode(), int(ncount), int(seed),
+                                      int(ntotalthreads or 0), int(threads_per_block or 0), int(flags), _dptr(g),
+                                      int(nscore), _dptr(grad), int(ndense),
+                                      _dptr(dense) if dense is not None else None, ctypes.byref(st)))
+        return grad, dense, st.as_dict()
 
     def read_dump(self):
         states = np.zeros((self._dump_count, 10))

@@ -23,14 +23,32 @@ class DeviceFunction:
     `rng`, `P` (packed parameters), `H` (histograms) and the launch block `A`.
     """
 
-    def __init__(self, call, stochastic=False, monitor=False, preamble=None):
+    def __init__(self, call, stochastic=False, monitor=False, preamble=None, adjoint=None, dparams=None,
+                 binning=None):
         self.call = call
         self.stochastic = stochastic
         self.monitor = monitor
         self.preamble = preamble
+        # adjoint kernels: `adjoint` is the score-tracking form of `call`
+        # ({s} = the component's score array, {dz} = the dense tracker),
+        # `dparams(comp)` lists (name, packed parameter index) in score-slot
+        # order, and a monitor's `binning` computes (bin, nbins) as `call` does
+        self.adjoint = adjoint
+        self.dparams = dparams
+        self.binning = binning
 
     def render(self, comp, i, n="n"):
         return self.call.format(i=i, n=n, **comp._render_context(i))
+
+    def render_adjoint(self, comp, i, s, n="n"):
+        return self.adjoint.format(i=i, n=n, s=s, dz="dz", **comp._render_context(i))
+
+    def render_binning(self, comp, i, n="n"):
+        return self.binning.format(i=i, n=n, **comp._render_context(i))
+
+    def differentiable(self, comp):
+        """(name, packed parameter index) of the weight-carrying parameters."""
+        return list(self.dparams(comp)) if self.dparams is not None else []
 
     def render_preamble(self, comp, i):
         if self.preamble is None:

@@ -78,7 +78,8 @@ class IQE_monitor(base):
             data=self.out_p * scale_factor,
             errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::iqe_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True)
+    propagate = DeviceFunction("mcv::iqe_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
+        binning="mcv::iqe_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         params, nQ, nE, _ = self.propagate_params

@@ -49,7 +49,8 @@ class TubeDetectorArray(base):
         return Histogram('I(tube,pixel,tof)', axes, data=self.out_p * scale_factor,
                          errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::tube_detector({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True)
+    propagate = DeviceFunction("mcv::tube_detector({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
+        binning="mcv::tube_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         params, nt, npx, ntof, _ = self.propagate_params

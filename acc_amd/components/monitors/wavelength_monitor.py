@@ -53,7 +53,8 @@ class Wavelength_monitor(base):
             data=self.out_p * scale_factor,
             errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::wavelength_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True)
+    propagate = DeviceFunction("mcv::wavelength_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
+        binning="mcv::wavelength_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         limits, nchan, _ = self.propagate_params

@@ -44,7 +44,10 @@ class Guide(ComponentBase):
             float(R0), float(Qc), float(alpha), float(m), float(W),
         )
 
-    propagate = DeviceFunction("mcv::guide({n}, P + A.off[{i}]);")
+    propagate = DeviceFunction(
+        "mcv::guide({n}, P + A.off[{i}]);",
+        adjoint="mcv::guide_t({n}, P + A.off[{i}], mcv::GuideScore({s}));",
+        dparams=lambda comp: [("R0", 5), ("Qc", 6), ("alpha", 7), ("m", 8), ("W", 9)])
 
     def _mcv_desc(self):
         return dict(type=self.mcv_type, params=np.array(self.propagate_params, dtype=float))

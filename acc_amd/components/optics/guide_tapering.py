@@ -81,7 +81,11 @@ class Guide(ComponentBase):
         w2 = np.asarray(w2[:-1], dtype=np.float64)
         return h1, h2, w1, w2
 
-    propagate = DeviceFunction("mcv::guide_tapering({n}, P + A.off[{i}], P + A.doff[{i}]);")
+    propagate = DeviceFunction(
+        "mcv::guide_tapering({n}, P + A.off[{i}], P + A.doff[{i}]);",
+        adjoint="mcv::guide_tapering_t({n}, P + A.off[{i}], P + A.doff[{i}], mcv::TaperScore({s}));",
+        dparams=lambda comp: [("R0", 1), ("Qcx", 2), ("Qcy", 3), ("alphax", 4), ("alphay", 5),
+                              ("mx", 6), ("my", 7), ("W", 8)])
 
     def _mcv_desc(self):
         nseg, scalars, data = self.propagate_params

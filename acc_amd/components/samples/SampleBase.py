@@ -38,6 +38,18 @@ class SampleBase(StochasticComponentBase):
         return dict(type=self.mcv_type, params=self._sample_params(), iparams=iparams,
                     data=np.concatenate([nodes, kdata]))
 
+    @staticmethod
+    def differentiable_params(comp):
+        """Score slots of a sample in the adjoint kernel: the absorption
+        coefficient (at 2200 m/s) and the scattering coefficient, then for the
+        phonon-DOS kernel its peaks A_k, c_k, w_k (packed P[7..15])."""
+        from ..._abi import MCV_KERNEL_PHONON_DOS
+        out = [("absorption_coefficient", 0), ("scattering_coefficient", 1)]
+        if comp.kernel.kernel_type == MCV_KERNEL_PHONON_DOS:
+            for k in range(3):
+                out += [("A%d" % (k + 1), 7 + 3 * k), ("c%d" % (k + 1), 8 + 3 * k), ("w%d" % (k + 1), 9 + 3 * k)]
+        return out
+
     def _render_context(self, i):
         return dict(shape=f"Shape{i}", kind=self.kernel.kernel_type)
 

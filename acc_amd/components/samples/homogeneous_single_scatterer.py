@@ -27,7 +27,10 @@ def factory(shape, kernel):
 
         propagate = DeviceFunction(
             "mcv::hss_propagate<{shape}, {kind}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}]);",
-            stochastic=True, preamble=SampleBase._shape_preamble)
+            stochastic=True, preamble=SampleBase._shape_preamble,
+            adjoint="mcv::hss_propagate_t<{shape}, {kind}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}], "
+                    "mcv::SampleScore({s}, &{dz}));",
+            dparams=SampleBase.differentiable_params)
 
     HomogeneousSingleScattererBase.shape = shape
     HomogeneousSingleScattererBase.kernel = kernel

@@ -124,16 +124,19 @@ def sqe_grid_kernel(**kw):
     return SQEGridKernel(S, Qmin, Qmax, Emin, Emax, absorption_coefficient=mu, scattering_coefficient=sigma)
 
 
-def phonon_dos_kernel(seed=0):
+def phonon_dos_kernel(seed=0, amplitudes=None, centers=(10.0, 25.0, 40.0), widths=None,
+                      sigma_scatt_barn=5.0, sigma_abs_barn=2.0):
     """Three Gaussian DOS peaks at 10/25/40 meV, widths U(1, 4) meV, amplitudes
-    U(0.5, 1.5), both from default_rng(seed)."""
+    U(0.5, 1.5), both from default_rng(seed) unless given."""
     from .kernels import PhononDOSKernel
     rng = np.random.default_rng(seed)
-    widths = rng.uniform(1.0, 4.0, 3)
-    amps = rng.uniform(0.5, 1.5, 3)
-    mu = AL_NUMBER_DENSITY * 2.0 * BARN
-    sigma = AL_NUMBER_DENSITY * 5.0 * BARN
-    return PhononDOSKernel(amps, [10.0, 25.0, 40.0], widths, 0.0, 12.0, -50.0, 50.0, mu, sigma)
+    w = rng.uniform(1.0, 4.0, 3)
+    a = rng.uniform(0.5, 1.5, 3)
+    widths = w if widths is None else widths
+    amplitudes = a if amplitudes is None else amplitudes
+    mu = AL_NUMBER_DENSITY * sigma_abs_barn * BARN
+    sigma = AL_NUMBER_DENSITY * sigma_scatt_barn * BARN
+    return PhononDOSKernel(amplitudes, list(centers), widths, 0.0, 12.0, -50.0, 50.0, mu, sigma)
 
 
 def tube_array(name="detector", ntubes=400, npixels=128, ntof=500, radius=3.0, height=1.28,
@@ -177,6 +180,43 @@ def c5_single():
                              min_angle_out_of_plane=-90, max_angle_out_of_plane=90),
                  position=(0, 0, 0), relativeTo=s)
     return instr
+
+
+def c5(Ei=60.0, m=2.0, W=0.003, alpha=6.07, **kernel):
+    """The differentiable-fit chain: one scan instance (c2 source at Ei +- 5%,
+    straight + tapered guide with a shared coating m/W/alpha) with the
+    c1-sized sphere holding the phonon-DOS kernel, then an I(Q,E) monitor over
+    all angles at the sample and the 250 x 250 PSD.  `kernel` goes to
+    phonon_dos_kernel (amplitudes, centers, widths, cross-sections)."""
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .components.optics.guide import Guide
+    from .components.optics.guide_tapering import Guide as TaperedGuide
+    from .geometry import Sphere
+    instr = new_instrument("c5")
+    instr.append(_source(Ei, 0.05 * Ei), position=(0, 0, 0))
+    instr.append(Guide('guide', w1=0.05, h1=0.05, w2=0.05, h2=0.05, l=10.,
+                       R0=0.99, Qc=0.0219, alpha=alpha, m=m, W=W), position=(0, 0, 10.0))
+    instr.append(TaperedGuide('taper', option="file=" + taper_file(), l=10., mx=m, my=m,
+                              R0=0.99, Qcx=0.0219, Qcy=0.0219, alphax=alpha, alphay=alpha, W=W),
+                 position=(0, 0, 20.0))
+    HSS = factory(shape=Sphere(0.01), kernel=phonon_dos_kernel(**kernel))
+    instr.append(HSS('sample'), position=(0, 0, 30.5))
+    instr.append(IQE_monitor('iqe', Ei=Ei, Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100,
+                             min_angle_in_plane=-180, max_angle_in_plane=180,
+                             min_angle_out_of_plane=-90, max_angle_out_of_plane=90),
+                 position=(0, 0, 30.5))
+    instr.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=250, ny=250), position=(0, 0, 31.5))
+    return instr
+
+
+# theta of the differentiable fit: the nine peak parameters, the two
+# cross-sections and the shared guide coating
+C5_PARAMS = (["sample.A1", "sample.c1", "sample.w1", "sample.A2", "sample.c2", "sample.w2",
+              "sample.A3", "sample.c3", "sample.w3", "sample.scattering_coefficient",
+              "sample.absorption_coefficient", ["guide.m", "taper.mx", "taper.my"], ["guide.W", "taper.W"],
+              ["guide.alpha", "taper.alphax", "taper.alphay"]])
 
 
 def c3(ntubes=400, npixels=128, ntof=500, max_ms_loops=2, max_ms_loops_path1=3, max_scattered_neutrons=10):
@@ -236,4 +276,4 @@ def scan(ninst=64, seed=0, nmin=1e5, nmax=1e7, sample="sphere"):
             for k, (e, r) in enumerate(zip(Ei, rot))], n
 
 
-BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3": c3, "c3s": c3_single, "c5s": c5_single}
+BUILDERS = {"c1": c1, "c2": c2, "c2s": c2s, "c3": c3, "c3s": c3_single, "c5s": c5_single, "c5": c5}

@@ -81,7 +81,8 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
 
 // ------------------------------------------------------------------ Guide
 // acc/components/optics/guide.py:59-162.  P = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W
-__device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) {
+template <class T>
+__device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P, T tr) {
   const double ww = P[0], hh = P[1], hw1 = P[2], hh1 = P[3], l = P[4];
   double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz, t = n.t, prob = n.p;
   const double dt = -z / vz;
@@ -134,6 +135,7 @@ __device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) 
     else vy = vy + dl;
     vz = vz - d * wh;
     const double R = calc_reflectivity(q, P[5], P[6], P[7], P[8], P[9]);
+    if (R > 0) tr.mirror(vert ? 0 : 1, q, P[5], P[6], P[7], P[8], P[9], 1.0);
     prob *= R;
     if (prob <= 0) break;
   }
@@ -142,12 +144,15 @@ __device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) 
   n.p = prob;
 }
 
+__device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) { guide_t(n, P, NoTrack{}); }
+
 // ------------------------------------------------------------------ Guide_tapering
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
 // D = 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf).
-__device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restrict__ P,
-                                               const double* __restrict__ D) {
+template <class T>
+__device__ __forceinline__ void guide_tapering_t(Neutron& n, const double* __restrict__ P,
+                                                 const double* __restrict__ D, T tr) {
   const double l_seg = P[0], R0 = P[1], W = P[8];
   const int nseg = (int)P[9];
   const double* __restrict__ w1 = D;
@@ -226,12 +231,17 @@ __device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restr
         x += hadj;
         break;
       }
-      double m = mxy;
-      if (m <= 0.0) m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
+      double m = mxy, dm = 1.0;
+      if (m <= 0.0) {
+        const double ratio = vert ? w1[0] / w1s : h1[0] / h1[seg];
+        m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
+        dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
+      }
       const double R = calc_reflectivity(q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W);
-      if (R > 1e-10)
+      if (R > 1e-10) {
+        tr.mirror(vert ? 0 : 1, q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W, dm);
         prob *= R;
-      else {
+      } else {
         n.p = 0;
         return;
       }
@@ -243,7 +253,15 @@ __device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restr
   n.p = prob;
 }
 
+__device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restrict__ P,
+                                               const double* __restrict__ D) {
+  guide_tapering_t(n, P, D, NoTrack{});
+}
+
 // ------------------------------------------------------------------ monitors
+// Each monitor is split into its binning (`*_bin`: the bin index, or -1 when
+// the neutron is not counted, and the number of bins) and the histogram
+// update, so the adjoint kernel bins exactly as the forward one does.
 // acc/neutron.py:83-89 prop_z0 (monitors do not write it back)
 __device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, double& t) {
   const double dt = -n.z / n.vz;
@@ -253,41 +271,58 @@ __device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, 
 }
 
 // acc/components/monitors/psd_monitor.py:79-96.  P = xmin, xmax, ymin, ymax, nx, ny
-__device__ __forceinline__ void psd_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+__device__ __forceinline__ int64_t psd_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t nx = (int64_t)P[4], ny = (int64_t)P[5];
+  nbins = nx * ny;
   double x, y, t;
   prop_z0(n, x, y, t);
-  if (n.t > t) return;
+  if (n.t > t) return -1;
   const double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3];
   // strict window as the reference; written so that NaN is rejected too
-  if (!(x > xmin && x < xmax && y > ymin && y < ymax)) return;
-  const int64_t nx = (int64_t)P[4], ny = (int64_t)P[5];
+  if (!(x > xmin && x < xmax && y > ymin && y < ymax)) return -1;
   int64_t ix = (int64_t)floor((x - xmin) / (xmax - xmin) * (double)nx);
   int64_t iy = (int64_t)floor((y - ymin) / (ymax - ymin) * (double)ny);
   ix = ix < 0 ? 0 : (ix < nx ? ix : nx - 1);
   iy = iy < 0 ? 0 : (iy < ny ? iy : ny - 1);
-  hist_add3(H, nx * ny, ix * ny + iy, n.p);
+  return ix * ny + iy;
+}
+
+__device__ __forceinline__ void psd_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+  int64_t nb;
+  const int64_t b = psd_bin(n, P, nb);
+  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 // acc/components/monitors/wavelength_monitor.py:149-171.  P = xmin..ymax, Lmin, Lmax, nchan
-__device__ __forceinline__ void wavelength_monitor(const Neutron& n, const double* __restrict__ P,
-                                                   double* __restrict__ H) {
+__device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double* __restrict__ P,
+                                                  int64_t& nbins) {
+  const int64_t nchan = (int64_t)P[6];
+  nbins = nchan;
   double x, y, t;
   prop_z0(n, x, y, t);
-  if (n.t > t) return;
-  if (!(x > P[0] && x < P[1] && y > P[2] && y < P[3])) return;
+  if (n.t > t) return -1;
+  if (!(x > P[0] && x < P[1] && y > P[2] && y < P[3])) return -1;
   const double v = sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz);
   const double L = 2 * PI / (v * V2K);
   const double Lmin = P[4], Lmax = P[5];
-  if (!(L > Lmin && L < Lmax)) return;
-  const int64_t nchan = (int64_t)P[6];
+  if (!(L > Lmin && L < Lmax)) return -1;
   int64_t iL = (int64_t)floor((L - Lmin) / (Lmax - Lmin) * (double)nchan);
   iL = iL < 0 ? 0 : (iL < nchan ? iL : nchan - 1);
-  hist_add3(H, nchan, iL, n.p);
+  return iL;
+}
+
+__device__ __forceinline__ void wavelength_monitor(const Neutron& n, const double* __restrict__ P,
+                                                   double* __restrict__ H) {
+  int64_t nb;
+  const int64_t b = wavelength_bin(n, P, nb);
+  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 // acc/components/monitors/iqe_monitor.py:251-284.
 // P = Ei, Qmin, Qmax, Emin, Emax, max_in, min_in, max_out, min_out, nQ, nE
-__device__ __forceinline__ void iqe_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+__device__ __forceinline__ int64_t iqe_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
+  nbins = nQ * nE;
   const double vx = n.vx, vy = n.vy, vz = n.vz;
   const double angle_in_plane = atan2(vx, vz) / PI * 180.;
   double angle_out_of_plane = 0.;
@@ -297,47 +332,59 @@ __device__ __forceinline__ void iqe_monitor(const Neutron& n, const double* __re
   }
   if (!(P[6] < angle_in_plane && angle_in_plane < P[5] && P[8] < angle_out_of_plane &&
         angle_out_of_plane < P[7]))
-    return;
+    return -1;
   const double Ei = P[0], Qmin = P[1], Qmax = P[2], Emin = P[3], Emax = P[4];
   const double vi = sqrt(Ei) * SE2V;
   const double Q = sqrt(vx * vx + vy * vy + (vz - vi) * (vz - vi)) * V2K;
   const double E = Ei - (vx * vx + vy * vy + vz * vz) * VS2E;
-  if (!((Q >= Qmin && Q < Qmax) && (E >= Emin && E < Emax))) return;
-  const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
+  if (!((Q >= Qmin && Q < Qmax) && (E >= Emin && E < Emax))) return -1;
   int64_t iQ = (int64_t)floor((Q - Qmin) / (Qmax - Qmin) * (double)nQ);
   int64_t iE = (int64_t)floor((E - Emin) / (Emax - Emin) * (double)nE);
   iQ = iQ < 0 ? 0 : (iQ < nQ ? iQ : nQ - 1);
   iE = iE < 0 ? 0 : (iE < nE ? iE : nE - 1);
-  hist_add3(H, nQ * nE, iQ * nE + iE, n.p);
+  return iQ * nE + iE;
+}
+
+__device__ __forceinline__ void iqe_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
+  int64_t nb;
+  const int64_t b = iqe_bin(n, P, nb);
+  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 // Cylindrical array of vertical tubes (no reference counterpart; monitor
 // conventions of psd_monitor.py).  P = radius, height, phi_min, phi_max (deg),
 // tof_min, tof_max (s), tube_diameter, unused, ntubes, npixels, ntof.
-__device__ __forceinline__ void tube_detector(const Neutron& n, const double* __restrict__ P,
-                                              double* __restrict__ H) {
+__device__ __forceinline__ int64_t tube_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t ntubes = (int64_t)P[8], npix = (int64_t)P[9], ntof = (int64_t)P[10];
+  nbins = ntubes * npix * ntof;
   const double R = P[0], Ht = P[1], phimin = P[2], phimax = P[3], tmin = P[4], tmax = P[5];
   const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
   const double a = vx * vx + vz * vz;
-  if (a == 0) return;
+  if (a == 0) return -1;
   const double b = x * vx + z * vz, c = x * x + z * z - R * R;
   const double k = b * b - a * c;
-  if (k < 0) return;
+  if (k < 0) return -1;
   const double dt = (-b + sqrt(k)) / a;
-  if (dt < 0) return;
+  if (dt < 0) return -1;
   const double xh = x + vx * dt, yh = y + vy * dt, zh = z + vz * dt, tof = n.t + dt;
   const double phi = atan2(xh, zh) / PI * 180.;
-  if (!(phi >= phimin && phi < phimax)) return;
-  if (!(yh > -Ht / 2 && yh < Ht / 2)) return;
-  if (!(tof >= tmin && tof < tmax)) return;
-  const int64_t ntubes = (int64_t)P[8], npix = (int64_t)P[9], ntof = (int64_t)P[10];
+  if (!(phi >= phimin && phi < phimax)) return -1;
+  if (!(yh > -Ht / 2 && yh < Ht / 2)) return -1;
+  if (!(tof >= tmin && tof < tmax)) return -1;
   int64_t it = (int64_t)floor((phi - phimin) / (phimax - phimin) * (double)ntubes);
   int64_t ip = (int64_t)floor((yh + Ht / 2) / Ht * (double)npix);
   int64_t ib = (int64_t)floor((tof - tmin) / (tmax - tmin) * (double)ntof);
   it = it < 0 ? 0 : (it < ntubes ? it : ntubes - 1);
   ip = ip < 0 ? 0 : (ip < npix ? ip : npix - 1);
   ib = ib < 0 ? 0 : (ib < ntof ? ib : ntof - 1);
-  hist_add3(H, ntubes * npix * ntof, (it * npix + ip) * ntof + ib, n.p);
+  return (it * npix + ip) * ntof + ib;
+}
+
+__device__ __forceinline__ void tube_detector(const Neutron& n, const double* __restrict__ P,
+                                              double* __restrict__ H) {
+  int64_t nb;
+  const int64_t b = tube_bin(n, P, nb);
+  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 }  // namespace mcv

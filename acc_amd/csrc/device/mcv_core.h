@@ -62,12 +62,33 @@ __device__ __forceinline__ void prop_dt_inplace(Neutron& n, double dt) {
 }
 
 // ------------------------------------------------------------------ histograms
-// Monitors own (3, nbins) float64 blocks: N, sum p, sum p^2.  Device memory
-// from hipMalloc is coarse-grained, so the FP64 add executes in L2.
+// Monitors own (3, nbins) float64 blocks: N, sum p, sum p^2, updated with
+// device-scope FP64 atomic adds that return nothing (the wave does not wait
+// for them).  With eight XCDs each keeping its own L2, a device-scope atomic
+// is performed past L2: rocprofv3 counts one 64-byte memory write per add.
 __device__ __forceinline__ void hist_add3(double* __restrict__ H, int64_t nbins, int64_t b, double p) {
   unsafeAtomicAdd(H + b, 1.0);
   unsafeAtomicAdd(H + nbins + b, p);
   unsafeAtomicAdd(H + 2 * nbins + b, p * p);
 }
+
+// ------------------------------------------------------------------ score tracking
+// Components that carry differentiable weight factors take a tracker; the
+// forward kernels pass NoTrack, whose hooks compile to nothing, so forward
+// and adjoint kernels share one copy of every component's arithmetic (and the
+// adjoint retraces exactly the forward paths).  The hooks:
+//   mirror(axis, q, R0, Qc, alpha, m, W, dm)  weight *= R(q; ...) > 0, with
+//                                             dm = d m / d(m parameter)
+//   attenuate(path, inv_v)                    weight *= exp(-(mu_a(v) + sigma) path)
+//   sigma()                                   weight *= sigma
+//   dos(kp, E)                                weight *= Q^2 sum_k A_k exp(-u_k^2 / 2)
+//   grid(i00, i10, i01, i11, fx, fy, S)       weight *= S (bilinear in the 4 nodes)
+struct NoTrack {
+  __device__ __forceinline__ void mirror(int, double, double, double, double, double, double, double) {}
+  __device__ __forceinline__ void attenuate(double, double) {}
+  __device__ __forceinline__ void sigma(double) {}
+  __device__ __forceinline__ void dos(const double*, double) {}
+  __device__ __forceinline__ void grid(int64_t, int64_t, int64_t, int64_t, double, double, double) {}
+};
 
 }  // namespace mcv

@@ -44,4 +44,11 @@ struct BatchArgs {
   int32_t ninst;
 };
 
+// Second argument of an adjoint kernel (mcv_adjoint.h).
+struct AdjointArgs {
+  const double* G;   // upstream gradient, laid out like the histograms (3 rows per monitor)
+  double* grad;      // (nscore) result, zeroed before the launch
+  double* dense;     // (ndense) gradient of a sample's S(Q,E) grid values, or null
+};
+
 }  // namespace mcv

@@ -377,7 +377,9 @@ __device__ __forceinline__ double absorption_coeff(const double* P, const Neutro
 }
 
 // acc/kernels/isotropic.py:5-21 with p *= sigma (acc/kernels/__init__.py:363-371)
-__device__ __forceinline__ void isotropic_scatter(Rng& rng, Neutron& n, double sigma) {
+template <class T>
+__device__ __forceinline__ void isotropic_scatter(Rng& rng, Neutron& n, double sigma, T& tr) {
+  tr.sigma(sigma);
   n.p *= sigma;
   double cos_t = rng.uniform() * 2 - 1;
   const double phi = rng.uniform() * (2 * PI);
@@ -391,19 +393,22 @@ __device__ __forceinline__ void isotropic_scatter(Rng& rng, Neutron& n, double s
   n.vz = vi * cos_t;
 }
 
-// bilinear grid lookup with the index clamping of acc/kernels/SANS2D_ongrid.py:16-56
+// bilinear grid lookup with the index clamping of acc/kernels/SANS2D_ongrid.py:16-56;
+// (ix, iy) is the lower node of the cell used and (fx, fy) the position in it.
 __device__ __forceinline__ double grid_interp(const double* __restrict__ g, int64_t nx, int64_t ny, double x,
-                                              double xmin, double xmax, double y, double ymin, double ymax) {
+                                              double xmin, double xmax, double y, double ymin, double ymax,
+                                              int64_t& ix, int64_t& iy, double& fx, double& fy) {
   const double dx = (xmax - xmin) / (double)nx, dy = (ymax - ymin) / (double)ny;
-  int64_t ix = (int64_t)((x - xmin) / dx);
-  int64_t iy = (int64_t)((y - ymin) / dy);
+  ix = (int64_t)((x - xmin) / dx);
+  iy = (int64_t)((y - ymin) / dy);
   ix = ix < 0 ? 0 : ix;
   iy = iy < 0 ? 0 : iy;
   ix = ix < nx - 2 ? ix : nx - 2;
   iy = iy < ny - 2 ? iy : ny - 2;
   const double f00 = g[ix * ny + iy], f10 = g[(ix + 1) * ny + iy], f01 = g[ix * ny + iy + 1],
                f11 = g[(ix + 1) * ny + iy + 1];
-  const double fx = (x - xmin) / dx - (double)ix, fy = (y - ymin) / dy - (double)iy;
+  fx = (x - xmin) / dx - (double)ix;
+  fy = (y - ymin) / dy - (double)iy;
   const double a00 = f00, a10 = f10 - f00, a01 = f01 - f00, a11 = f11 - f10 - f01 + f00;
   return a00 + a10 * fx + a01 * fy + a11 * fx * fy;
 }
@@ -422,9 +427,10 @@ __device__ __forceinline__ double dos_S(const double* __restrict__ kp, double Q,
 // S(Q,E) kernels (no reference): E_Q-style kinematic closure
 // (acc/kernels/E_Q.py:37-99) with Q and E both drawn uniformly, one draw, p = 0
 // when kinematically forbidden.  KIND = MCV_KERNEL_SQE_GRID (2) or PHONON_DOS (3).
-template <int KIND>
+template <int KIND, class T>
 __device__ __forceinline__ void sqe_scatter(Rng& rng, Neutron& n, const double* __restrict__ P,
-                                            const double* __restrict__ D) {
+                                            const double* __restrict__ D, T& tr) {
+  tr.sigma(P[1]);
   n.p *= P[1];
   const double* kp = P + 3;
   const double Qmin = kp[0], Qmax = kp[1], Emin = kp[2], Emax = kp[3];
@@ -468,28 +474,41 @@ __device__ __forceinline__ void sqe_scatter(Rng& rng, Neutron& n, const double* 
   n.vy = e1y * v1 + e2y * v2 + e3y * v3;
   n.vz = e1z * v1 + e2z * v2 + e3z * v3;
   double Sv;
-  if constexpr (KIND == 2)
-    Sv = grid_interp(D + (int64_t)P[21], (int64_t)P[22], (int64_t)P[23], Q, Qmin, Qmax, E, Emin, Emax);
-  else
+  if constexpr (KIND == 2) {
+    const int64_t n0 = (int64_t)P[22], n1 = (int64_t)P[23];
+    int64_t ix, iy;
+    double fx, fy;
+    Sv = grid_interp(D + (int64_t)P[21], n0, n1, Q, Qmin, Qmax, E, Emin, Emax, ix, iy, fx, fy);
+    tr.grid(ix * n1 + iy, (ix + 1) * n1 + iy, ix * n1 + iy + 1, (ix + 1) * n1 + iy + 1, fx, fy, Sv);
+  } else {
     Sv = dos_S(kp, Q, E);
+    tr.dos(kp, E);
+  }
   const double prob = Sv * (vf / vi) * Q * (Qmax - Qmin) * (Emax - Emin) / (kf * ki) / 2;
   n.p *= prob;
+}
+
+template <int KIND, class T>
+__device__ __forceinline__ void kernel_scatter(Rng& rng, Neutron& n, const double* __restrict__ P,
+                                               const double* __restrict__ D, T& tr) {
+  if constexpr (KIND == 1)
+    isotropic_scatter(rng, n, P[1], tr);
+  else
+    sqe_scatter<KIND>(rng, n, P, D, tr);
 }
 
 template <int KIND>
 __device__ __forceinline__ void kernel_scatter(Rng& rng, Neutron& n, const double* __restrict__ P,
                                                const double* __restrict__ D) {
-  if constexpr (KIND == 1)
-    isotropic_scatter(rng, n, P[1]);
-  else
-    sqe_scatter<KIND>(rng, n, P, D);
+  NoTrack tr;
+  kernel_scatter<KIND>(rng, n, P, D, tr);
 }
 
 // ---------------------------------------------------------------- HSS
 // acc/components/samples/homogeneous_single_scatterer.py:59-100
-template <class Shape, int KIND>
-__device__ __forceinline__ void hss_propagate(Rng& rng, Neutron& n, const double* __restrict__ P,
-                                              const double* __restrict__ D) {
+template <class Shape, int KIND, class T>
+__device__ __forceinline__ void hss_propagate_t(Rng& rng, Neutron& n, const double* __restrict__ P,
+                                                const double* __restrict__ D, T tr) {
   constexpr int K = Shape::K;
   double ts[K];
   const double vx = n.vx, vy = n.vy, vz = n.vz;
@@ -511,14 +530,22 @@ __device__ __forceinline__ void hss_propagate(Rng& rng, Neutron& n, const double
   const double sigma = P[1];
   double mu = absorption_coeff(P, n);
   const double atten = exp(-(mu + sigma) * dist);
+  tr.attenuate(dist, 1.0 / v);
   n.p *= fulllen * atten;
-  kernel_scatter<KIND>(rng, n, P, D);
+  kernel_scatter<KIND>(rng, n, P, D, tr);
   if (n.p <= 0) return;
   ninter = Shape::intersect(D, n.x, n.y, n.z, n.vx, n.vy, n.vz, ts);
   const double dt3 = total_time_in_shape<K>(ts, ninter);
   prop_dt_inplace(n, dt3);
   mu = absorption_coeff(P, n);
+  tr.attenuate(v * dt3, 1.0 / sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz));
   n.p *= exp(-(mu + sigma) * v * dt3);
+}
+
+template <class Shape, int KIND>
+__device__ __forceinline__ void hss_propagate(Rng& rng, Neutron& n, const double* __restrict__ P,
+                                              const double* __restrict__ D) {
+  hss_propagate_t<Shape, KIND>(rng, n, P, D, NoTrack{});
 }
 
 }  // namespace mcv

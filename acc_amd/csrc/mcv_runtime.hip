@@ -400,6 +400,16 @@ struct mcv_instr {
   hipFunction_t fn = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // adjoint kernel (loaded on first use)
+  hipModule_t adj_mod = nullptr;
+  hipFunction_t adj_fn = nullptr;
+  std::string adj_key;
+  int adj_block = 256;
+  double* d_G = nullptr;
+  double* d_grad = nullptr;
+  int64_t grad_cap = 0;
+  double* d_dense = nullptr;
+  int64_t dense_cap = 0;
 };
 
 namespace {
@@ -433,6 +443,10 @@ void release(mcv_instr* h) {
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   if (h->mod) (void)hipModuleUnload(h->mod);
+  if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
+  if (h->d_G) (void)hipFree(h->d_G);
+  if (h->d_grad) (void)hipFree(h->d_grad);
+  if (h->d_dense) (void)hipFree(h->d_dense);
   delete h;
 }
 
@@ -606,6 +620,120 @@ int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count) {
   HIPCHECK(hipMalloc(&h->d_alive, (size_t)count * sizeof(int32_t)));
   HIPCHECK(hipMemset(h->d_dump, 0, (size_t)count * 10 * sizeof(double)));
   HIPCHECK(hipMemset(h->d_alive, 0xff, (size_t)count * sizeof(int32_t)));
+  return MCV_OK;
+}
+
+int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_name, uint64_t ncount,
+                      uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags, const double* g_hist,
+                      int32_t nscore, double* grad, int64_t ndense, double* grad_dense, mcv_run_stats* stats) {
+  if (!h || !code_object || !kernel_name || !g_hist || nscore < 1 || !grad)
+    return fail(MCV_ERR_ARG, "bad adjoint arguments");
+  if (ndense < 0 || (ndense > 0 && !grad_dense)) return fail(MCV_ERR_ARG, "bad dense gradient buffer");
+  // the dense gradient is the S(Q,E) grid of the chain's grid-kernel sample: its size must match
+  int64_t grid_nodes = 0;
+  for (int c = 0; c < h->ncomp; ++c)
+    if (h->types[c] == MCV_SAMPLE_HSS && (int64_t)h->blob[h->args.off[c] + 17] == MCV_KERNEL_SQE_GRID)
+      grid_nodes = (int64_t)h->blob[h->args.off[c] + 22] * (int64_t)h->blob[h->args.off[c] + 23];
+  if (ndense != 0 && ndense != grid_nodes)
+    return fail(MCV_ERR_ARG, "dense gradient of %lld values for a grid of %lld nodes", (long long)ndense,
+                (long long)grid_nodes);
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::string key = std::string(code_object) + "\n" + kernel_name;
+  if (key != h->adj_key) {
+    if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
+    h->adj_mod = nullptr;
+    h->adj_fn = nullptr;
+    h->adj_key.clear();
+    hipError_t e = hipModuleLoad(&h->adj_mod, code_object);
+    if (e != hipSuccess) {
+      h->adj_mod = nullptr;
+      return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
+    }
+    if (hipModuleGetFunction(&h->adj_fn, h->adj_mod, kernel_name) != hipSuccess)
+      return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
+    int rc = module_block(h->adj_mod, &h->adj_block);
+    if (rc) return rc;
+    h->adj_key = key;
+  }
+  int64_t nblocks, nthreads, npt;
+  partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->nblocks = nblocks;
+    stats->nthreads = nthreads;
+    stats->npt = npt;
+  }
+  if (!h->d_G) HIPCHECK(hipMalloc(&h->d_G, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
+  if (nscore > h->grad_cap) {
+    if (h->d_grad) HIPCHECK(hipFree(h->d_grad));
+    h->d_grad = nullptr;
+    HIPCHECK(hipMalloc(&h->d_grad, (size_t)nscore * sizeof(double)));
+    h->grad_cap = nscore;
+  }
+  if (ndense > h->dense_cap) {
+    if (h->d_dense) HIPCHECK(hipFree(h->d_dense));
+    h->d_dense = nullptr;
+    HIPCHECK(hipMalloc(&h->d_dense, (size_t)ndense * sizeof(double)));
+    h->dense_cap = ndense;
+  }
+  if (nthreads > h->rng_cap) {
+    if (h->d_rng) HIPCHECK(hipFree(h->d_rng));
+    h->d_rng = nullptr;
+    HIPCHECK(hipMalloc(&h->d_rng, (size_t)nthreads * 2 * sizeof(uint64_t)));
+    h->rng_cap = nthreads;
+  }
+  if (h->hist_total > 0)
+    HIPCHECK(hipMemcpyAsync(h->d_G, g_hist, h->hist_total * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(mcv_zero_f64, dim3(1), dim3(256), 0, h->stream, h->d_grad, (int64_t)nscore);
+  if (ndense > 0)
+    hipLaunchKernelGGL(mcv_zero_f64, dim3((unsigned)std::min<int64_t>((ndense + 255) / 256, 4096)), dim3(256), 0,
+                       h->stream, h->d_dense, ndense);
+  hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, h->stream, h->d_counter, (int64_t)1);
+  HIPCHECK(hipGetLastError());
+  if (ncount > 0) {
+    HIPCHECK(hipEventRecord(h->ev[0], h->stream));
+    int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(h->ev[1], h->stream));
+    mcv::LaunchArgs A = h->args;
+    A.P = h->d_blob;
+    A.H = nullptr;
+    A.rng = h->d_rng;
+    A.counter = h->d_counter;
+    A.dump = nullptr;
+    A.dump_alive = nullptr;
+    A.dump_count = 0;
+    A.N = (int64_t)ncount;
+    A.nthreads = nthreads;
+    A.npt = npt;
+    A.tid_base = 0;
+    A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+    mcv::AdjointArgs J{h->d_G, h->d_grad, ndense > 0 ? h->d_dense : nullptr};
+    const unsigned bs = (unsigned)h->adj_block;
+    void* kargs[] = {&A, &J};
+    HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)((nthreads + bs - 1) / bs), 1, 1, bs, 1, 1, 0, h->stream,
+                                   kargs, nullptr));
+    HIPCHECK(hipEventRecord(h->ev[2], h->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  unsigned long long processed = 0;
+  HIPCHECK(hipMemcpy(&processed, h->d_counter, sizeof(processed), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(grad, h->d_grad, (size_t)nscore * sizeof(double), hipMemcpyDeviceToHost));
+  if (ndense > 0) HIPCHECK(hipMemcpy(grad_dense, h->d_dense, (size_t)ndense * sizeof(double), hipMemcpyDeviceToHost));
+  if (stats) {
+    stats->processed = processed;
+    if (ncount > 0) {
+      float ms_rng = 0, ms_k = 0;
+      HIPCHECK(hipEventElapsedTime(&ms_rng, h->ev[0], h->ev[1]));
+      HIPCHECK(hipEventElapsedTime(&ms_k, h->ev[1], h->ev[2]));
+      stats->ms_rng_init = ms_rng;
+      stats->ms_kernel = ms_k;
+    }
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (processed != ncount)
+    return fail(MCV_ERR_COUNT, "Processed neutron count %llu does not match requested neutron count %llu",
+                (unsigned long long)processed, (unsigned long long)ncount);
   return MCV_OK;
 }
 

@@ -120,6 +120,34 @@ def _iqe_normalizer():
     return i
 
 
+def grid_sphere(S=None):
+    """Source (30 +- 10 meV) -> sphere with an S(Q,E)-grid kernel (`S`, default
+    the 40 x 40 ridge grid) -> I(Q,E) monitor over all angles: the chain of the
+    dense-gradient test."""
+    from .instrument import instrument
+    from .configs import _source, sqe_grid_kernel, sqe_ridge_grid
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .geometry import Sphere
+    from .kernels import SQEGridKernel
+    S0, (Qmin, Qmax, Emin, Emax) = sqe_ridge_grid(nQ=40, nE=40)
+    S = S0 if S is None else S
+    k = sqe_grid_kernel()
+    kern = SQEGridKernel(S, Qmin, Qmax, Emin, Emax, k.absorption_coefficient, k.scattering_coefficient)
+    i = instrument("grid-sphere")
+    i.append(_source(30.0, 10.0), position=(0, 0, 0))
+    HSS = factory(shape=Sphere(0.01), kernel=kern)
+    i.append(HSS('sample'), position=(0, 0, 10.0))
+    i.append(IQE_monitor('iqe', Ei=30., Qmin=0., Qmax=10., nQ=50, Emin=-40., Emax=40., nE=40,
+                         min_angle_in_plane=-180, max_angle_in_plane=180,
+                         min_angle_out_of_plane=-90, max_angle_out_of_plane=90), position=(0, 0, 10.0))
+    return i
+
+
+# chains whose adjoint kernels are built ahead of time
+ADJOINT = ("c5", "grid-sphere")
+
+
 def instruments():
     from . import configs
     return {
@@ -136,6 +164,8 @@ def instruments():
         "iqe-normalizer": _iqe_normalizer,
         "scan": lambda: configs.scan_instance(60.0, 30.0),
         "scan-plate": lambda: configs.scan_instance(60.0, 30.0, sample="plate"),
+        "c5": configs.c5,
+        "grid-sphere": grid_sphere,
     }
 
 
@@ -147,7 +177,10 @@ def build_all(verbose=False, jobs=None):
     """Compile every listed chain (hipcc runs in parallel, one per job)."""
     from concurrent.futures import ThreadPoolExecutor
     from .fuser import generate_source, compile_source
+    from .fuser_adj import generate_adjoint_source
     srcs = {name: generate_source(make())[0] for name, make in instruments().items()}
+    for name in ADJOINT:
+        srcs[name + "@adjoint"] = generate_adjoint_source(instruments()[name]())[0]
     from .autotune import DEFAULT_BLOCKS, DEFAULT_WAVES
     for name in TUNED:
         for block in DEFAULT_BLOCKS:

@@ -178,6 +178,21 @@ int mcv_monitor_clear(mcv_instr* h);
 int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count);
 int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive);
 
+/* Vector-Jacobian product of the monitor histograms.  No reference
+ * counterpart (the reference only simulates forward).  `code_object` /
+ * `kernel_name` name the chain's adjoint kernel (generated by the fuser's
+ * adjoint mode for the same chain); it re-runs the same ncount neutrons with
+ * the same seed and partition as mcv_instr_run and returns, in grad[nscore],
+ *   sum_i sum_monitors (G[1][bin] p_i + 2 G[2][bin] p_i^2) d log p_i / d theta
+ * for the chain's weight-carrying parameters theta in score-slot order, with
+ * g_hist laid out like the histograms (hist_total doubles: per monitor rows
+ * N, sum p, sum p^2; the N row is ignored).  grad_dense (ndense = the node
+ * count of the chain's S(Q,E)-grid sample, or 0) receives the gradient with
+ * respect to that grid's values. */
+int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_name, uint64_t ncount,
+                      uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags, const double* g_hist,
+                      int32_t nscore, double* grad, int64_t ndense, double* grad_dense, mcv_run_stats* stats);
+
 /* Seed n xoroshiro128+ streams exactly as numba's create_xoroshiro128p_states
  * (SplitMix64(seed) into both words, stream i = stream i-1 jumped 2^64) and copy
  * the (n, 2) uint64 states to host memory.  Used to pin the device seeding. */

@@ -94,6 +94,8 @@ CASES = [
     ("c2s", True, 200000, 4096),
     ("c2s", False, 100000, 4096),
     ("scan-plate", True, 200000, 1000000),
+    ("c5", True, 150000, 4096),
+    ("grid-sphere", True, 150000, 4096),
     ("hollow-cylinder-hss", True, 300000, 8192),
     ("verdi-taper", True, 60000, 2048),
     ("c3s-small", True, 300000, 8192),
