@@ -170,7 +170,7 @@ def instruments():
 
 
 # chains also built for every launch shape the autotuner tries
-TUNED = ("c2", "c2s")
+TUNED = ("c2", "c2s", "c5")
 
 
 def build_all(verbose=False, jobs=None):

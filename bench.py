@@ -138,6 +138,49 @@ def batch_scan(ninst=64, reps=3):
             "note": "same tuned kernel in all three; back-to-back = one launch per instance, no host work between"}
 
 
+def adjoint_cost(ncount=int(1e7), reps=3):
+    """Adjoint (vector-Jacobian product) vs forward kernel time on the
+    differentiable-fit chain: c5 (source at 60 meV -> straight + tapered guide
+    -> sphere with the 3-peak phonon-DOS kernel -> I(Q,E) + PSD), 14
+    parameters (9 peak parameters, 2 cross-sections, shared m/W/alpha),
+    1e7 neutrons, the forward kernel at its autotuned launch shape."""
+    import numpy as np
+    from acc_amd import configs
+    from acc_amd.adjoint import DifferentiableRun
+    from acc_amd.autotune import tune_instrument
+    best = tune_instrument(configs.c5(), ncount, use_cache=False)
+    run = DifferentiableRun(configs.c5(), configs.C5_PARAMS, ncount=ncount, seed=0, block=best["block"],
+                            waves_per_eu=best["waves_per_eu"])
+    theta = run.theta0()
+    g = np.random.default_rng(0).uniform(0.5, 1.5, sum(run.sizes))
+    run.forward(theta)
+    run.vjp(g, theta)
+    fwd, adj = [], []
+    for _ in range(reps):
+        run.forward(theta)
+        fwd.append(run.last_forward_stats["ms_kernel"])
+        run.vjp(g, theta)
+        adj.append(run.last_adjoint_stats["ms_kernel"])
+    f, b = min(fwd), min(adj)
+    return {"workload": "c5 differentiable fit chain, 14 parameters", "ncount": ncount,
+            "ms_forward_kernel": f, "ms_adjoint_kernel": b, "adjoint_forward_ratio": b / f,
+            "forward_launch_shape": {"block": best["block"], "waves_per_eu": best["waves_per_eu"]},
+            "check": "adjoint vs CRN central differences (h=1e-6 rel) of forward kernel and CPU oracle"}
+
+
+def copy_kernel(nbytes=1 << 31, reps=3):
+    """A plain device copy of known size: rocprofv3's FETCH_SIZE / WRITE_SIZE
+    for it give the factor between those counters and bytes moved."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel_bytes_read": nbytes, "copy_kernel_bytes_written": nbytes, "reps": reps}))
+    return 0
+
+
 def traffic_from_profiles(workload):
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
@@ -157,7 +200,13 @@ def main():
     ap.add_argument("--falg-neutrons", type=int, default=1000000)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--shape", default=None, help="fixed launch shape BLOCK/WAVES (e.g. 512/None) instead of tuning")
+    ap.add_argument("--copy-kernel", action="store_true",
+                    help="only run a plain 2 GiB device copy 3 times (HBM counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_kernel:
+        return copy_kernel()
     if a.gpus != 1:
         print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
               "(multi-GPU sharding by stream ranges is described in DESIGN.md); use --gpus 1", file=sys.stderr)
@@ -171,7 +220,12 @@ def main():
     desc, default_n = WORKLOADS[a.workload]
     ncount = int(a.ncount or default_n)
     instr = configs.BUILDERS[a.workload]()
-    if a.no_tune:
+    if a.shape:
+        blk, wv = a.shape.split("/")
+        wv = None if wv in ("None", "", "0") else int(wv)
+        h, shape = get_handle(instr, block=int(blk), waves_per_eu=wv), {"block": int(blk), "waves_per_eu": wv,
+                                                                         "tuned": False}
+    elif a.no_tune:
         h, shape = get_handle(instr), {"block": None, "waves_per_eu": None, "tuned": False}
     else:
         h, best = tuned_handle(instr, ncount, use_cache=False)
@@ -212,6 +266,8 @@ def main():
     }
     if not a.no_batch:
         line["batch_scan"] = batch_scan()
+    if not a.no_adjoint:
+        line["adjoint"] = adjoint_cost()
     print(json.dumps(line))
     return 0
 
