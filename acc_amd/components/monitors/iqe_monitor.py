@@ -78,8 +78,10 @@ class IQE_monitor(base):
             data=self.out_p * scale_factor,
             errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::iqe_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
-        binning="mcv::iqe_bin({n}, P + A.off[{i}], nb)")
+    propagate = DeviceFunction(
+        "{{ int64_t nb; const int64_t b = mcv::iqe_bin({n}, P + A.off[{i}], nb);"
+        " if (b >= 0) mcv::hist_count(A, {i}, nb, b, {n}.p); }}",
+        monitor=True, binning="mcv::iqe_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         params, nQ, nE, _ = self.propagate_params

@@ -59,8 +59,10 @@ class PSD_monitor(base):
             data=self.out_p * scale_factor,
             errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::psd_monitor({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
-        binning="mcv::psd_bin({n}, P + A.off[{i}], nb)")
+    propagate = DeviceFunction(
+        "{{ int64_t nb; const int64_t b = mcv::psd_bin({n}, P + A.off[{i}], nb);"
+        " if (b >= 0) mcv::hist_count(A, {i}, nb, b, {n}.p); }}",
+        monitor=True, binning="mcv::psd_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         limits, nx, ny, _ = self.propagate_params

@@ -49,8 +49,10 @@ class TubeDetectorArray(base):
         return Histogram('I(tube,pixel,tof)', axes, data=self.out_p * scale_factor,
                          errors=self.out_p2 * scale_factor * scale_factor)
 
-    propagate = DeviceFunction("mcv::tube_detector({n}, P + A.off[{i}], H + A.hoff[{i}]);", monitor=True,
-        binning="mcv::tube_bin({n}, P + A.off[{i}], nb)")
+    propagate = DeviceFunction(
+        "{{ int64_t nb; const int64_t b = mcv::tube_bin({n}, P + A.off[{i}], nb);"
+        " if (b >= 0) mcv::hist_count(A, {i}, nb, b, {n}.p); }}",
+        monitor=True, binning="mcv::tube_bin({n}, P + A.off[{i}], nb)")
 
     def _mcv_desc(self):
         params, nt, npx, ntof, _ = self.propagate_params

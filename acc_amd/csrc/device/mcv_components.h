@@ -259,9 +259,9 @@ __device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restr
 }
 
 // ------------------------------------------------------------------ monitors
-// Each monitor is split into its binning (`*_bin`: the bin index, or -1 when
-// the neutron is not counted, and the number of bins) and the histogram
-// update, so the adjoint kernel bins exactly as the forward one does.
+// Each monitor is its binning (`*_bin`: the bin index, or -1 when the neutron
+// is not counted, and the number of bins); the fused kernel adds (1, p, p^2)
+// to the bin (mcv::hist_count) and the adjoint kernel weighs the same bin.
 // acc/neutron.py:83-89 prop_z0 (monitors do not write it back)
 __device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, double& t) {
   const double dt = -n.z / n.vz;
@@ -287,12 +287,6 @@ __device__ __forceinline__ int64_t psd_bin(const Neutron& n, const double* __res
   return ix * ny + iy;
 }
 
-__device__ __forceinline__ void psd_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
-  int64_t nb;
-  const int64_t b = psd_bin(n, P, nb);
-  if (b >= 0) hist_add3(H, nb, b, n.p);
-}
-
 // acc/components/monitors/wavelength_monitor.py:149-171.  P = xmin..ymax, Lmin, Lmax, nchan
 __device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double* __restrict__ P,
                                                   int64_t& nbins) {
@@ -309,13 +303,6 @@ __device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double
   int64_t iL = (int64_t)floor((L - Lmin) / (Lmax - Lmin) * (double)nchan);
   iL = iL < 0 ? 0 : (iL < nchan ? iL : nchan - 1);
   return iL;
-}
-
-__device__ __forceinline__ void wavelength_monitor(const Neutron& n, const double* __restrict__ P,
-                                                   double* __restrict__ H) {
-  int64_t nb;
-  const int64_t b = wavelength_bin(n, P, nb);
-  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 // acc/components/monitors/iqe_monitor.py:251-284.
@@ -345,12 +332,6 @@ __device__ __forceinline__ int64_t iqe_bin(const Neutron& n, const double* __res
   return iQ * nE + iE;
 }
 
-__device__ __forceinline__ void iqe_monitor(const Neutron& n, const double* __restrict__ P, double* __restrict__ H) {
-  int64_t nb;
-  const int64_t b = iqe_bin(n, P, nb);
-  if (b >= 0) hist_add3(H, nb, b, n.p);
-}
-
 // Cylindrical array of vertical tubes (no reference counterpart; monitor
 // conventions of psd_monitor.py).  P = radius, height, phi_min, phi_max (deg),
 // tof_min, tof_max (s), tube_diameter, unused, ntubes, npixels, ntof.
@@ -378,13 +359,6 @@ __device__ __forceinline__ int64_t tube_bin(const Neutron& n, const double* __re
   ip = ip < 0 ? 0 : (ip < npix ? ip : npix - 1);
   ib = ib < 0 ? 0 : (ib < ntof ? ib : ntof - 1);
   return (it * npix + ip) * ntof + ib;
-}
-
-__device__ __forceinline__ void tube_detector(const Neutron& n, const double* __restrict__ P,
-                                              double* __restrict__ H) {
-  int64_t nb;
-  const int64_t b = tube_bin(n, P, nb);
-  if (b >= 0) hist_add3(H, nb, b, n.p);
 }
 
 }  // namespace mcv

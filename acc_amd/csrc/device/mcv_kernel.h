@@ -53,6 +53,46 @@ __device__ __forceinline__ void dump_neutron(const LaunchArgs& A, int64_t idx, c
   }
 }
 
+// ------------------------------------------------------------------ monitor histograms
+// Small monitor histograms live in LDS for the lifetime of a block of streams
+// (LDS FP64 atomics), and are added to the global histograms with one
+// device-scope atomic per non-empty bin when the block is done; the rest go
+// straight to global memory.
+extern __shared__ double mcv_lds_hist[];
+
+__device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_t nbins, int64_t b, double p) {
+  const int lo = A.loff[comp];
+  if (lo >= 0) {
+    double* L = mcv_lds_hist + lo;
+    atomicAdd(L + b, 1.0);
+    atomicAdd(L + nbins + b, p);
+    atomicAdd(L + 2 * nbins + b, p * p);
+  } else {
+    hist_add3(A.H + A.hoff[comp], nbins, b, p);
+  }
+}
+
+// All threads of the workgroup call these (barriers when A.lds_nseg > 0).
+__device__ __forceinline__ void lds_hist_zero(const LaunchArgs& A) {
+  if (A.lds_nseg == 0) return;
+  for (int j = threadIdx.x; j < A.lds_total; j += blockDim.x) mcv_lds_hist[j] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void lds_hist_flush(const LaunchArgs& A) {
+  if (A.lds_nseg == 0) return;
+  __syncthreads();
+  for (int k = 0; k < A.lds_nseg; ++k) {
+    const double* L = mcv_lds_hist + A.lds_loff[k];
+    double* G = A.H + A.lds_hoff[k];
+    for (int j = threadIdx.x; j < A.lds_len[k]; j += blockDim.x) {
+      const double v = L[j];
+      if (v != 0) unsafeAtomicAdd(G + j, v);
+    }
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void count_processed(unsigned long long* counter, int64_t mine) {
   // wavefront reduction, one atomic per wave
   unsigned long long v = (unsigned long long)mine;
@@ -80,6 +120,7 @@ __device__ __forceinline__ void stream_range(const LaunchArgs& A, int64_t tid, R
 // chains, the number of outputs.
 template <class Chain>
 __device__ __forceinline__ void run_lane(const LaunchArgs& A, int64_t tid) {
+  lds_hist_zero(A);
   Rng rng;
   int64_t idx, end;
   stream_range(A, tid, rng, idx, end);
@@ -91,6 +132,7 @@ __device__ __forceinline__ void run_lane(const LaunchArgs& A, int64_t tid) {
     dump_neutron(A, i, n, alive);
   }
   count_processed(A.counter, end - idx);
+  lds_hist_flush(A);
 }
 
 // ------------------------------------------------------------------ compacted chains
@@ -178,6 +220,7 @@ __device__ __forceinline__ void round_totals(Q& q, int& par, bool has, bool more
 template <class Chain>
 __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t tid, QueueLDS& q) {
   const int lt = threadIdx.x;
+  lds_hist_zero(A);
   if (lt == 0) q.count = 0;
   Rng rng;
   int64_t idx, end;
@@ -219,6 +262,7 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
     }
   }
   count_processed(A.counter, mine);
+  lds_hist_flush(A);
   __syncthreads();  // the queue is reused by the next block of streams
 }
 
@@ -231,6 +275,7 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
 template <class Chain>
 __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q) {
   const int lt = threadIdx.x;
+  lds_hist_zero(A);
   if (lt == 0) q.count = 0;
   q.returned[lt] = 0;
   Rng rng;
@@ -294,6 +339,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     }
   }
   count_processed(A.counter, mine);
+  lds_hist_flush(A);
   __syncthreads();
 }
 

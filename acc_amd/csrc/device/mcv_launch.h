@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #define MCV_MAX_COMP 32
+#define MCV_LDS_SEG 8  // monitors whose histograms a workgroup keeps in LDS
 
 namespace mcv {
 
@@ -27,6 +28,13 @@ struct LaunchArgs {
   int64_t doff[MCV_MAX_COMP];        // data table of component i
   int64_t hoff[MCV_MAX_COMP];        // (3, nbins) histogram of monitor i
   int64_t xoff[MCV_MAX_COMP];        // frame change i -> i+1: R[9], o[3]
+  // Monitor histograms privatised in LDS (dynamic shared memory of lds_total
+  // doubles): loff[i] = LDS offset of monitor i's (3, nbins) block or -1;
+  // segment k copies LDS [lds_loff[k], +lds_len[k]) onto H + lds_hoff[k].
+  int32_t loff[MCV_MAX_COMP];
+  int32_t lds_nseg, lds_total;
+  int32_t lds_loff[MCV_LDS_SEG], lds_len[MCV_LDS_SEG];
+  int64_t lds_hoff[MCV_LDS_SEG];
 };
 
 // Persistent kernel of a batched scan: instances of one chain structure with

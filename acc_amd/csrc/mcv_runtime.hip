@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -329,6 +330,16 @@ int64_t align8(int64_t n) { return (n + 7) & ~int64_t(7); }
 
 // Pack one instrument: blob = frame changes, then per-component parameters,
 // then data tables (64-byte aligned blocks); offsets go into `args`.
+// LDS budget (doubles) for privatised monitor histograms per workgroup:
+// MCV_LDS_HIST_DOUBLES from the environment (0 turns privatisation off),
+// default 2048 (16 KB).
+int64_t lds_hist_budget() {
+  const char* e = std::getenv("MCV_LDS_HIST_DOUBLES");
+  if (!e || !*e) return 2048;
+  const long long v = std::atoll(e);
+  return v < 0 ? 0 : std::min<long long>(v, 8192);
+}
+
 int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
                  mcv::LaunchArgs& args, std::vector<double>& B, std::vector<int64_t>& nbins, int64_t& hist_total,
                  std::vector<int>& types, std::vector<int>& nparams) {
@@ -362,6 +373,21 @@ int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offset
     args.hoff[i] = hist_total;
     nbins.push_back(packed[i].nbins);
     hist_total += 3 * packed[i].nbins;
+  }
+  // monitors kept in LDS by each workgroup: in chain order while they fit the budget
+  const int64_t budget = lds_hist_budget();
+  args.lds_nseg = 0;
+  args.lds_total = 0;
+  for (int i = 0; i < MCV_MAX_COMP; ++i) args.loff[i] = -1;
+  for (int i = 0; i < ncomp; ++i) {
+    const int64_t len = 3 * packed[i].nbins;
+    if (len <= 0 || args.lds_nseg >= MCV_LDS_SEG || args.lds_total + len > budget) continue;
+    args.loff[i] = args.lds_total;
+    args.lds_loff[args.lds_nseg] = args.lds_total;
+    args.lds_len[args.lds_nseg] = (int32_t)len;
+    args.lds_hoff[args.lds_nseg] = args.hoff[i];
+    args.lds_nseg += 1;
+    args.lds_total += (int32_t)len;
   }
   args.ncomp = ncomp;
   return MCV_OK;
@@ -573,7 +599,8 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   const unsigned bs = (unsigned)h->block;
   const int64_t grid = (nthreads + bs - 1) / bs;
   void* kargs[] = {&A};
-  HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, 0, h->stream, kargs, nullptr));
+  HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, (unsigned)(A.lds_total * sizeof(double)),
+                                 h->stream, kargs, nullptr));
   HIPCHECK(hipEventRecord(h->ev[2], h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   unsigned long long processed = 0;
@@ -708,6 +735,8 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     A.npt = npt;
     A.tid_base = 0;
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+    A.lds_nseg = 0;  // the adjoint kernel fills no histograms
+    A.lds_total = 0;
     mcv::AdjointArgs J{h->d_G, h->d_grad, ndense > 0 ? h->d_dense : nullptr};
     const unsigned bs = (unsigned)h->adj_block;
     void* kargs[] = {&A, &J};
@@ -903,8 +932,8 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.ninst = b->ninst;
   const int64_t g = std::min<int64_t>(grid > 0 ? grid : 1024, std::max<int64_t>(total_items, 1));
   void* kargs[] = {&B};
-  HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, (unsigned)b->block, 1, 1, 0, b->stream, kargs,
-                                 nullptr));
+  HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, (unsigned)b->block, 1, 1,
+                                 (unsigned)(B.base.lds_total * sizeof(double)), b->stream, kargs, nullptr));
   return MCV_OK;
 }
 
@@ -1037,7 +1066,7 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
     HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + b->block - 1) / b->block), 1, 1, (unsigned)b->block, 1,
-                                   1, 0, b->stream, kargs,
+                                   1, (unsigned)(A.lds_total * sizeof(double)), b->stream, kargs,
                                    nullptr));
     HIPCHECK(hipEventRecord(b->ev[1], b->stream));
     HIPCHECK(hipStreamSynchronize(b->stream));
