@@ -31,6 +31,7 @@ MCV_MON_WAVELENGTH = 11
 MCV_MON_IQE = 12
 MCV_DET_TUBES = 13
 MCV_SRC_IQE_NORMALIZER = 20
+MCV_SRC_STORAGE = 21
 
 MCV_KERNEL_ISOTROPIC = 1
 MCV_KERNEL_SQE_GRID = 2
@@ -51,7 +52,7 @@ EXPORTED = (
     "mcv_instr_set_params", "mcv_instr_run", "mcv_monitor_read", "mcv_monitor_clear",
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
-    "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint",
+    "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process",
 )
 
 
@@ -144,6 +145,10 @@ def lib():
     L.mcv_batch_clear.argtypes = [ctypes.c_void_p]
     L.mcv_batch_destroy.restype = None
     L.mcv_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.mcv_instr_process.restype = ctypes.c_int
+    L.mcv_instr_process.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, dp, ctypes.c_int64,
+                                    ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, dp,
+                                    ctypes.c_int32, ctypes.POINTER(McvRunStats)]
     L.mcv_instr_adjoint.restype = ctypes.c_int
     L.mcv_instr_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
                                     ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, dp,
@@ -248,6 +253,17 @@ class Handle:
                                       int(nscore), _dptr(grad), int(ndense),
                                       _dptr(dense) if dense is not None else None, ctypes.byref(st)))
         return grad, dense, st.as_dict()
+
+    def process(self, code_object, kernel_name, neutrons, seed, m=0, ntotalthreads=0, threads_per_block=0,
+                flags=MCV_FLAG_GATE_DEAD):
+        """mcv_instr_process: (rows after, or the (n*m, 10) outputs when m > 0; run stats)."""
+        arr = np.array(neutrons, dtype=np.float64, order="C").reshape(-1, 10)
+        out = np.zeros((arr.shape[0] * m, 10)) if m else None
+        st = McvRunStats()
+        check(lib().mcv_instr_process(self.h, code_object.encode(), kernel_name.encode(), _dptr(arr), arr.shape[0],
+                                      int(seed), int(ntotalthreads or 0), int(threads_per_block or 0), int(flags),
+                                      _dptr(out) if out is not None else None, int(m), ctypes.byref(st)))
+        return (out if m else arr), st.as_dict()
 
     def read_dump(self):
         states = np.zeros((self._dump_count, 10))

@@ -137,7 +137,7 @@ __device__ __forceinline__ void dense_add(const AdjointArgs& J, const DenseTrack
   }
 }
 
-// Chain::body(rng, n, A, J, acc) runs one neutron and adds its monitor
+// Chain::body(rng, n, A, J, acc, nidx) runs neutron nidx and adds its monitor
 // contributions to acc[0 .. Chain::NS).
 template <class Chain>
 __device__ __forceinline__ void run_lane_adjoint(const LaunchArgs& A, const AdjointArgs& J, int64_t tid) {
@@ -150,7 +150,7 @@ __device__ __forceinline__ void run_lane_adjoint(const LaunchArgs& A, const Adjo
   Neutron n;
   n.s1 = 0;
   n.s2 = 0;
-  for (int64_t i = idx; i < end; ++i) Chain::body(rng, n, A, J, acc);
+  for (int64_t i = idx; i < end; ++i) Chain::body(rng, n, A, J, acc, i);
   count_processed(A.counter, end - idx);
 #pragma unroll
   for (int k = 0; k < Chain::NS; ++k) {

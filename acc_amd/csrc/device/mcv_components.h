@@ -365,6 +365,17 @@ __device__ __forceinline__ int64_t tube_bin(const Neutron& n, const double* __re
 
 namespace mcv {
 
+// ------------------------------------------------------------------ stored neutrons
+// acc/components/sources/neutronfromstorage.py:70-82: neutron i of the run is
+// row i % N of the stored (N, 10) array.  P[0] = N, D = the rows.
+__device__ __forceinline__ void storage_source(Neutron& n, const double* __restrict__ P,
+                                               const double* __restrict__ D, int64_t nidx) {
+  const int64_t N = (int64_t)P[0];
+  const double* r = D + 10 * (nidx % N);
+  n.x = r[0]; n.y = r[1]; n.z = r[2]; n.vx = r[3]; n.vy = r[4]; n.vz = r[5];
+  n.s1 = r[6]; n.s2 = r[7]; n.t = r[8]; n.p = r[9];
+}
+
 // ------------------------------------------------------------------ IQE normalizer source
 // Neutrons of the I(Q,E) monitor's Monte-Carlo normalizer
 // (acc/components/monitors/iqe_monitor.py:317-344, makeNeutrons): energy

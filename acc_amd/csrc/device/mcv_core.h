@@ -20,6 +20,7 @@ constexpr double PI = 3.141592653589793;
 struct Neutron {
   double x, y, z, vx, vy, vz, s1, s2, t, p;
 };
+static_assert(sizeof(Neutron) == 10 * sizeof(double), "a neutron is one 10-double row");
 
 // ------------------------------------------------------------------ RNG
 struct Rng {

@@ -115,7 +115,7 @@ __device__ __forceinline__ void stream_range(const LaunchArgs& A, int64_t tid, R
 }
 
 // ------------------------------------------------------------------ plain chains
-// Chain::body(Rng&, Neutron&, const LaunchArgs&) returns the dump flag: 1/0
+// Chain::body(Rng&, Neutron&, const LaunchArgs&, int64_t nidx) returns the dump flag: 1/0
 // whether the neutron reached the end alive, or, for multiple-scattering
 // chains, the number of outputs.
 template <class Chain>
@@ -128,11 +128,55 @@ __device__ __forceinline__ void run_lane(const LaunchArgs& A, int64_t tid) {
   n.s1 = 0;
   n.s2 = 0;
   for (int64_t i = idx; i < end; ++i) {
-    const int alive = (int)Chain::body(rng, n, A);
+    const int alive = (int)Chain::body(rng, n, A, i);
     dump_neutron(A, i, n, alive);
   }
   count_processed(A.counter, end - idx);
   lds_hist_flush(A);
+}
+
+// ------------------------------------------------------------------ buffered mode
+// acc/components/ComponentBase.py:79-107: the chain runs over the rows of a
+// neutron array in place, with the partition of the array's length.
+__device__ __forceinline__ void load_row(const double* __restrict__ r, Neutron& n) {
+  n.x = r[0]; n.y = r[1]; n.z = r[2]; n.vx = r[3]; n.vy = r[4]; n.vz = r[5];
+  n.s1 = r[6]; n.s2 = r[7]; n.t = r[8]; n.p = r[9];
+}
+
+__device__ __forceinline__ void store_row(double* __restrict__ r, const Neutron& n) {
+  r[0] = n.x; r[1] = n.y; r[2] = n.z; r[3] = n.vx; r[4] = n.vy; r[5] = n.vz;
+  r[6] = n.s1; r[7] = n.s2; r[8] = n.t; r[9] = n.p;
+}
+
+template <class Chain>
+__device__ __forceinline__ void run_lane_process(const LaunchArgs& A, int64_t tid) {
+  lds_hist_zero(A);
+  Rng rng;
+  int64_t idx, end;
+  stream_range(A, tid, rng, idx, end);
+  for (int64_t i = idx; i < end; ++i) {
+    Neutron n;
+    load_row(A.io + 10 * i, n);
+    Chain::body(rng, n, A, i);
+    store_row(A.io + 10 * i, n);
+  }
+  count_processed(A.counter, end - idx);
+  lds_hist_flush(A);
+}
+
+// Chain::scatter(rng, n, A, out) of a lone multiple-scattering sample: up to
+// A.io_m outputs, the unused rows are written as zeros.
+template <class Chain>
+__device__ __forceinline__ void run_lane_process_ms(const LaunchArgs& A, int64_t tid) {
+  Rng rng;
+  int64_t idx, end;
+  stream_range(A, tid, rng, idx, end);
+  for (int64_t i = idx; i < end; ++i) {
+    Neutron n;
+    load_row(A.io + 10 * i, n);
+    Chain::scatter(rng, n, A, A.io_out + 10 * (int64_t)A.io_m * i);
+  }
+  count_processed(A.counter, end - idx);
 }
 
 // ------------------------------------------------------------------ compacted chains
@@ -235,7 +279,7 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
     n.s2 = 0;
     const int64_t my = idx;
     if (idx < end) {
-      has = Chain::phase_a(rng, n, A) != 0;
+      has = Chain::phase_a(rng, n, A, my) != 0;
       if (!has) dump_neutron(A, my, n, 0);
       ++idx;
     }
@@ -292,7 +336,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     n.s2 = 0;
     const int64_t my = idx;
     if (!busy && idx < end) {
-      has = Chain::phase_a(rng, n, A) != 0;
+      has = Chain::phase_a(rng, n, A, my) != 0;
       if (!has) dump_neutron(A, my, n, 0);
       ++idx;
     }
@@ -408,4 +452,16 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \
     while (mcv::next_item(B, A, tid)) mcv::run_block_compacted_rng<CHAIN>(A, tid, q);         \
+  }
+
+#define MCV_DEFINE_PROCESS_KERNEL(NAME, CHAIN)                                                 \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
+    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
+    mcv::run_lane_process<CHAIN>(A, tid);                                                     \
+  }
+
+#define MCV_DEFINE_PROCESS_MS_KERNEL(NAME, CHAIN)                                              \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
+    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
+    mcv::run_lane_process_ms<CHAIN>(A, tid);                                                  \
   }

@@ -35,6 +35,12 @@ struct LaunchArgs {
   int32_t lds_nseg, lds_total;
   int32_t lds_loff[MCV_LDS_SEG], lds_len[MCV_LDS_SEG];
   int64_t lds_hoff[MCV_LDS_SEG];
+  // buffered mode: neutron i of the run is row i of io (N, 10), updated in
+  // place; a multiple-scattering sample writes its io_m outputs per neutron to
+  // io_out rows [i*io_m, (i+1)*io_m)
+  double* io;
+  double* io_out;
+  int32_t io_m;
 };
 
 // Persistent kernel of a batched scan: instances of one chain structure with

@@ -155,9 +155,15 @@ __global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restri
   out[2 * i + 1] = b;
 }
 
+// Per-device state shared by every handle: the jump tables and one stream.
+// All handles of a device queue their work on that one stream (calls are
+// synchronous anyway): the runtime reserves a kernel's scratch (private
+// memory) per hardware queue, and the multiple-scattering kernels need tens of
+// KB of it per lane, so one queue per handle multiplied those reservations.
 struct DeviceCtx {
   int device = -1;
   uint64_t* d_mats = nullptr;
+  hipStream_t stream = nullptr;
 };
 
 std::mutex g_ctx_mu;
@@ -177,6 +183,7 @@ int device_ctx(DeviceCtx** out) {
   const auto& flat = jump_matrices();
   HIPCHECK(hipMalloc(&c.d_mats, flat.size() * sizeof(uint64_t)));
   HIPCHECK(hipMemcpy(c.d_mats, flat.data(), flat.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  HIPCHECK(hipStreamCreate(&c.stream));
   g_ctx.push_back(c);
   *out = &g_ctx.back();
   return MCV_OK;
@@ -255,6 +262,13 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
     case MCV_SRC_IQE_NORMALIZER:  // Ei, Emin, Emax
       if ((rc = expect(d, i, 3, 0))) return rc;
       copyp(3);
+      return MCV_OK;
+    case MCV_SRC_STORAGE:  // P[0] = N; data = (N, 10)
+      if ((rc = expect(d, i, 0, 1))) return rc;
+      if (d.iparams[0] < 1 || d.n_data != 10 * d.iparams[0] || !d.data)
+        return fail(MCV_ERR_ARG, "component %d: stored neutrons need (N >= 1, 10) data", i);
+      out.params.push_back((double)d.iparams[0]);
+      out.data.assign(d.data, d.data + d.n_data);
       return MCV_OK;
     case MCV_GUIDE:
       if ((rc = expect(d, i, 10, 0))) return rc;
@@ -436,6 +450,15 @@ struct mcv_instr {
   int64_t grad_cap = 0;
   double* d_dense = nullptr;
   int64_t dense_cap = 0;
+  // buffered-mode kernel (loaded on first use)
+  hipModule_t proc_mod = nullptr;
+  hipFunction_t proc_fn = nullptr;
+  std::string proc_key;
+  int proc_block = 256;
+  double* d_io = nullptr;
+  int64_t io_cap = 0;
+  double* d_io_out = nullptr;
+  int64_t io_out_cap = 0;
 };
 
 namespace {
@@ -467,9 +490,11 @@ void release(mcv_instr* h) {
   if (h->d_alive) (void)hipFree(h->d_alive);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
   if (h->mod) (void)hipModuleUnload(h->mod);
   if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
+  if (h->proc_mod) (void)hipModuleUnload(h->proc_mod);
+  if (h->d_io) (void)hipFree(h->d_io);
+  if (h->d_io_out) (void)hipFree(h->d_io_out);
   if (h->d_G) (void)hipFree(h->d_G);
   if (h->d_grad) (void)hipFree(h->d_grad);
   if (h->d_dense) (void)hipFree(h->d_dense);
@@ -493,7 +518,12 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   if (rc) return rc;
   std::vector<double>& B = h->blob;
 
-  HIPCHECK(hipStreamCreate(&h->stream));
+  {
+    DeviceCtx* ctx = nullptr;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    h->stream = ctx->stream;  // shared, not owned
+  }
   for (auto& e : h->ev) HIPCHECK(hipEventCreate(&e));
   HIPCHECK(hipMalloc(&h->d_blob, B.size() * sizeof(double)));
   HIPCHECK(hipMemcpy(h->d_blob, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -668,6 +698,9 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
   const std::string key = std::string(code_object) + "\n" + kernel_name;
   if (key != h->adj_key) {
     if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
+  if (h->proc_mod) (void)hipModuleUnload(h->proc_mod);
+  if (h->d_io) (void)hipFree(h->d_io);
+  if (h->d_io_out) (void)hipFree(h->d_io_out);
     h->adj_mod = nullptr;
     h->adj_fn = nullptr;
     h->adj_key.clear();
@@ -766,6 +799,110 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
   return MCV_OK;
 }
 
+int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_name, double* neutrons, int64_t n,
+                      uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags, double* out_ms,
+                      int32_t m, mcv_run_stats* stats) {
+  if (!h || !code_object || !kernel_name || n < 0 || (n > 0 && !neutrons) || m < 0 || (m > 0 && !out_ms))
+    return fail(MCV_ERR_ARG, "bad buffered-mode arguments");
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::string key = std::string(code_object) + "\n" + kernel_name;
+  if (key != h->proc_key) {
+    if (h->proc_mod) (void)hipModuleUnload(h->proc_mod);
+    h->proc_mod = nullptr;
+    h->proc_fn = nullptr;
+    h->proc_key.clear();
+    hipError_t e = hipModuleLoad(&h->proc_mod, code_object);
+    if (e != hipSuccess) {
+      h->proc_mod = nullptr;
+      return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
+    }
+    if (hipModuleGetFunction(&h->proc_fn, h->proc_mod, kernel_name) != hipSuccess)
+      return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
+    int rc = module_block(h->proc_mod, &h->proc_block);
+    if (rc) return rc;
+    h->proc_key = key;
+  }
+  int64_t nblocks, nthreads, npt;
+  partition((uint64_t)n, ntotalthreads, tpb, nblocks, nthreads, npt);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->nblocks = nblocks;
+    stats->nthreads = nthreads;
+    stats->npt = npt;
+  }
+  if (n == 0) return MCV_OK;
+  if (n > h->io_cap) {
+    if (h->d_io) HIPCHECK(hipFree(h->d_io));
+    h->d_io = nullptr;
+    HIPCHECK(hipMalloc(&h->d_io, (size_t)n * 10 * sizeof(double)));
+    h->io_cap = n;
+  }
+  if (m > 0 && n * m > h->io_out_cap) {
+    if (h->d_io_out) HIPCHECK(hipFree(h->d_io_out));
+    h->d_io_out = nullptr;
+    HIPCHECK(hipMalloc(&h->d_io_out, (size_t)n * m * 10 * sizeof(double)));
+    h->io_out_cap = n * m;
+  }
+  if (nthreads > h->rng_cap) {
+    if (h->d_rng) HIPCHECK(hipFree(h->d_rng));
+    h->d_rng = nullptr;
+    HIPCHECK(hipMalloc(&h->d_rng, (size_t)nthreads * 2 * sizeof(uint64_t)));
+    h->rng_cap = nthreads;
+  }
+  HIPCHECK(hipMemcpyAsync(h->d_io, neutrons, (size_t)n * 10 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, h->stream, h->d_counter, (int64_t)1);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(h->ev[0], h->stream));
+  int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord(h->ev[1], h->stream));
+  mcv::LaunchArgs A = h->args;
+  A.P = h->d_blob;
+  A.H = h->d_hist;
+  A.rng = h->d_rng;
+  A.counter = h->d_counter;
+  A.dump = nullptr;
+  A.dump_alive = nullptr;
+  A.dump_count = 0;
+  A.N = n;
+  A.nthreads = nthreads;
+  A.npt = npt;
+  A.tid_base = 0;
+  A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+  A.io = h->d_io;
+  A.io_out = m > 0 ? h->d_io_out : nullptr;
+  A.io_m = m;
+  if (m > 0) {  // the multiple-scattering process kernel keeps no histograms
+    A.lds_nseg = 0;
+    A.lds_total = 0;
+  }
+  const unsigned bs = (unsigned)h->proc_block;
+  void* kargs[] = {&A};
+  HIPCHECK(hipModuleLaunchKernel(h->proc_fn, (unsigned)((nthreads + bs - 1) / bs), 1, 1, bs, 1, 1,
+                                 (unsigned)(A.lds_total * sizeof(double)), h->stream, kargs, nullptr));
+  HIPCHECK(hipEventRecord(h->ev[2], h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  unsigned long long processed = 0;
+  HIPCHECK(hipMemcpy(&processed, h->d_counter, sizeof(processed), hipMemcpyDeviceToHost));
+  if (m > 0)
+    HIPCHECK(hipMemcpy(out_ms, h->d_io_out, (size_t)n * m * 10 * sizeof(double), hipMemcpyDeviceToHost));
+  else
+    HIPCHECK(hipMemcpy(neutrons, h->d_io, (size_t)n * 10 * sizeof(double), hipMemcpyDeviceToHost));
+  if (stats) {
+    float ms_rng = 0, ms_k = 0;
+    HIPCHECK(hipEventElapsedTime(&ms_rng, h->ev[0], h->ev[1]));
+    HIPCHECK(hipEventElapsedTime(&ms_k, h->ev[1], h->ev[2]));
+    stats->processed = processed;
+    stats->ms_rng_init = ms_rng;
+    stats->ms_kernel = ms_k;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if ((int64_t)processed != n)
+    return fail(MCV_ERR_COUNT, "Processed neutron count %llu does not match requested neutron count %lld",
+                (unsigned long long)processed, (long long)n);
+  return MCV_OK;
+}
+
 int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive) {
   if (!h || !h->d_dump) return fail(MCV_ERR_ARG, "no dump configured");
   HIPCHECK(hipMemcpy(states, h->d_dump, (size_t)h->dump_count * 10 * sizeof(double), hipMemcpyDeviceToHost));
@@ -828,7 +965,6 @@ void release_batch(mcv_batch* b) {
   if (b->d_counters) (void)hipFree(b->d_counters);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
   if (b->mod) (void)hipModuleUnload(b->mod);
   delete b;
 }
@@ -869,7 +1005,12 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
     all.insert(all.end(), blob.begin(), blob.end());
   }
   b->hstride = hist_total;
-  HIPCHECK(hipStreamCreate(&b->stream));
+  {
+    DeviceCtx* ctx = nullptr;
+    int rc = device_ctx(&ctx);
+    if (rc) return rc;
+    b->stream = ctx->stream;  // shared, not owned
+  }
   for (auto& e : b->ev) HIPCHECK(hipEventCreate(&e));
   HIPCHECK(hipMalloc(&b->d_blob, all.size() * sizeof(double)));
   HIPCHECK(hipMemcpy(b->d_blob, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -58,16 +58,17 @@ def chain_body(components):
 def split_index(components):
     """Index of the last component of phase A (the per-stream part) of the
     compacted kernel, or None.  Phase A ends at the first component after the
-    first drawing one (the source) that is neither a monitor nor an Arm: the
+    source (or the first drawing component) that is neither a monitor nor an Arm: the
     first point where neutrons die in numbers (a guide entrance, a sample).
     Compaction needs a phase B holding a component that is not a monitor
     (otherwise there is nothing to run densely)."""
     from .components.optics.arm import Arm
-    stoch = [i for i, c in enumerate(components) if c.propagate.stochastic]
-    if not stoch:
+    from .components.sources.SourceBase import SourceBase
+    first = [i for i, c in enumerate(components) if c.propagate.stochastic or isinstance(c, SourceBase)]
+    if not first:
         return None
     k = None
-    for j in range(stoch[0] + 1, len(components)):
+    for j in range(first[0] + 1, len(components)):
         c = components[j]
         if not c.propagate.monitor and not isinstance(c, Arm):
             k = j
@@ -142,10 +143,11 @@ def _generate(instrument, compact):
 namespace {{
 {pre}
 struct Chain {{
-  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                                 const int64_t nidx) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  (void)rng; (void)H;
+  (void)rng; (void)H; (void)nidx;
   {a}
   }}
 
@@ -169,10 +171,11 @@ struct Chain {{
 namespace {{
 {pre}
 struct Chain {{
-  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                              const int64_t nidx) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  (void)rng; (void)H;
+  (void)rng; (void)H; (void)nidx;
   {body}
   }}
 }};

@@ -77,8 +77,9 @@ namespace {{
 struct AdjChain {{
   static constexpr int NS = {ns};
   __device__ static __forceinline__ void body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
-                                               const mcv::AdjointArgs& J, double* acc) {{
+                                               const mcv::AdjointArgs& J, double* acc, const int64_t nidx) {{
   const double* __restrict__ P = A.P;
+  (void)nidx;
   {body}
   }}
 }};

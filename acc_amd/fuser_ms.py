@@ -53,10 +53,11 @@ def generate_ms_source(instrument):
 namespace {{
 {pre}
 struct Chain {{
-  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int body(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                              const int64_t nidx) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  (void)rng; (void)H;
+  (void)rng; (void)H; (void)nidx;
   {body}
   }}
 }};

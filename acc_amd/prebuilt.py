@@ -148,6 +148,45 @@ def grid_sphere(S=None):
 ADJOINT = ("c5", "grid-sphere")
 
 
+def _storage_psd():
+    """Stored-neutron source (the reference's singletestneutron2.mcv) -> PSD."""
+    from .instrument import instrument
+    from .components.sources.neutronfromstorage import NeutronFromStorage
+    from .components.monitors.psd_monitor import PSD_monitor
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    i = instrument("storage-psd")
+    i.append(NeutronFromStorage('src', os.path.join(here, "tests", "golden", "sources", "singletestneutron2.mcv")),
+             position=(0, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=5, ny=5), position=(0, 0, 1.0))
+    return i
+
+
+def single(comp):
+    """A one-component chain (the buffered-mode unit)."""
+    from .instrument import instrument
+    i = instrument("process-" + comp.name)
+    i.append(comp, position=(0, 0, 0))
+    return i
+
+
+def process_units():
+    """Components the buffered-mode tests run alone."""
+    from . import configs
+    from .components.optics.guide import Guide
+    from .components.monitors.psd_monitor import PSD_monitor
+    c2 = configs.c2()
+    hms = _hms_reference()
+    return {
+        "source": lambda: single(configs._source(60.0, 10.0)),
+        "guide": lambda: single(Guide('guide', w1=0.05, h1=0.05, w2=0.03, h2=0.03, l=5.)),
+        "taper": lambda: single(c2.components[2]),
+        "psd": lambda: single(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=20, ny=20)),
+        "hms": lambda: single(hms.components[1]),
+        "storage": lambda: single(_storage_psd().components[0]),
+        "c2-chain": configs.c2,
+    }
+
+
 def instruments():
     from . import configs
     return {
@@ -166,6 +205,7 @@ def instruments():
         "scan-plate": lambda: configs.scan_instance(60.0, 30.0, sample="plate"),
         "c5": configs.c5,
         "grid-sphere": grid_sphere,
+        "storage-psd": _storage_psd,
     }
 
 
@@ -181,6 +221,9 @@ def build_all(verbose=False, jobs=None):
     srcs = {name: generate_source(make())[0] for name, make in instruments().items()}
     for name in ADJOINT:
         srcs[name + "@adjoint"] = generate_adjoint_source(instruments()[name]())[0]
+    from .fuser_proc import generate_process_source
+    for name, make in process_units().items():
+        srcs[name + "@process"] = generate_process_source(make())[0]
     from .autotune import DEFAULT_BLOCKS, DEFAULT_WAVES
     for name in TUNED:
         for block in DEFAULT_BLOCKS:

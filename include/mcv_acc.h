@@ -87,6 +87,8 @@ extern "C" {
 #define MCV_SRC_IQE_NORMALIZER 20 /* params[3] = Ei, Emin, Emax: the I(Q,E) monitor's
                                      Monte-Carlo normalizer neutrons
                                      (acc/components/monitors/iqe_monitor.py:317-344) */
+#define MCV_SRC_STORAGE 21        /* iparams[1] = N; data = (N, 10) stored neutrons: neutron i is
+                                     row i % N (acc/components/sources/neutronfromstorage.py:70-82) */
 
 /* scattering kernels of samples (acc/kernels/__init__.py:8-22,363-371) */
 #define MCV_KERNEL_ISOTROPIC 1  /* acc/kernels/isotropic.py:5-21 */
@@ -177,6 +179,21 @@ int mcv_monitor_clear(mcv_instr* h);
  * with a per-neutron flag (1 = reached the end of the chain alive). */
 int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count);
 int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive);
+
+/* Buffered mode (acc/components/ComponentBase.py:79-107 `process`, launched as
+ * acc/components/StochasticComponentBase.py:21-55 does): `code_object` /
+ * `kernel_name` name the chain's buffered-mode kernel (the fuser's process
+ * mode for the same chain).  neutrons (n x 10 doubles, x y z vx vy vz s1 s2 t p)
+ * are run through the chain in place, stream tid taking rows
+ * [tid*npt, min((tid+1)*npt, n)) of the partition of n, streams seeded from
+ * `seed`; monitors add to the handle's histograms (not cleared).  For a lone
+ * multiple-scattering sample (m = its max_scattered_neutrons > 0) the rows
+ * are inputs only and out_ms (n*m x 10) receives neutron i's outputs in rows
+ * [i*m, i*m + num) with zeros after them.  Rows are not filtered: the
+ * reference's p > 0 selection is the caller's. */
+int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_name, double* neutrons, int64_t n,
+                      uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags, double* out_ms,
+                      int32_t m, mcv_run_stats* stats);
 
 /* Vector-Jacobian product of the monitor histograms.  No reference
  * counterpart (the reference only simulates forward).  `code_object` /

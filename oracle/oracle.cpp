@@ -1430,10 +1430,17 @@ struct Chain {
   bool gate;
 };
 
-static void run_component(const Chain& C, int i, Rng& rng, double* n, double* hist) {
+// acc/components/sources/neutronfromstorage.py:70-82
+static void storage_source(double* n, const int64_t* IP, const double* data, int64_t nidx) {
+  const int64_t N = IP[0];
+  std::memcpy(n, data + 10 * (nidx % N), 10 * sizeof(double));
+}
+
+static void run_component(const Chain& C, int i, Rng& rng, double* n, double* hist, int64_t nidx) {
   const Comp& c = C.comps[i];
   switch (c.type) {
     case MCV_ARM: break;
+    case MCV_SRC_STORAGE: storage_source(n, c.IP, c.data, nidx); break;
     case MCV_SOURCE_SIMPLE: source_simple(rng, n, c.P); break;
     case MCV_SRC_IQE_NORMALIZER: iqe_normalizer_source(rng, n, c.P); break;
     case MCV_GUIDE: guide(n, c.P); break;
@@ -1451,11 +1458,11 @@ static inline bool dead(const Chain& C, const double* n) { return C.gate && !(n[
 
 // Components [from, end) on one neutron (the tail of the fused kernel body,
 // acc/run_script.py:197-231).  Returns true if it reached the end alive.
-static bool run_tail(const Chain& C, int from, Rng& rng, double* n, double* hist) {
+static bool run_tail(const Chain& C, int from, Rng& rng, double* n, double* hist, int64_t nidx) {
   int nc = (int)C.comps.size();
   for (int i = from; i < nc; ++i) {
     if (i > 0) apply_transformation(n, C.rotmats + 9 * (i - 1), C.offsets + 3 * (i - 1));
-    run_component(C, i, rng, n, hist);
+    run_component(C, i, rng, n, hist, nidx);
     if (dead(C, n)) return false;
   }
   return true;
@@ -1466,16 +1473,16 @@ static bool run_tail(const Chain& C, int from, Rng& rng, double* n, double* hist
 // the chain (run_script.py:225-230); n then holds output 0's final state and
 // the return value is the number of outputs.
 static int run_neutron(const Chain& C, Rng& rng, double* n, double* hist, std::vector<double>& msbuf,
-                       bool& alive) {
+                       bool& alive, int64_t nidx) {
   int nc = (int)C.comps.size();
   if (C.ms_index < 0) {
-    alive = run_tail(C, 0, rng, n, hist);
+    alive = run_tail(C, 0, rng, n, hist, nidx);
     return 1;
   }
   int ms = C.ms_index;
   for (int i = 0; i < ms; ++i) {
     if (i > 0) apply_transformation(n, C.rotmats + 9 * (i - 1), C.offsets + 3 * (i - 1));
-    run_component(C, i, rng, n, hist);
+    run_component(C, i, rng, n, hist, nidx);
     if (dead(C, n)) {
       alive = false;
       return 0;
@@ -1493,7 +1500,7 @@ static int run_neutron(const Chain& C, Rng& rng, double* n, double* hist, std::v
     if (dead(C, o))
       a = false;
     else if (ms + 1 < nc)
-      a = run_tail(C, ms + 1, rng, o, hist);
+      a = run_tail(C, ms + 1, rng, o, hist, nidx);
     if (k == 0) {
       clone(o, n);
       alive = a;
@@ -1692,11 +1699,59 @@ int64_t orc_run(const mcv_comp_desc* comps, int ncomp, const double* offsets, co
       for (int64_t idx = start; idx < end; ++idx) {
         processed += 1;
         bool alive = true;
-        int num = run_neutron(C, rng, n, h, msbuf, alive);
+        int num = run_neutron(C, rng, n, h, msbuf, alive, idx);
         if (dump_states && idx >= dump_first && idx < dump_first + dump_count) {
           int64_t k = idx - dump_first;
           std::memcpy(dump_states + 10 * k, n, 10 * sizeof(double));
           dump_alive[k] = C.ms_index >= 0 ? num : (alive ? 1 : 0);
+        }
+      }
+    }
+  }
+  for (int t = 0; t < nt; ++t)
+    for (int64_t k = 0; k < C.hist_total; ++k) hist[k] += hists[t][k];
+  return processed;
+}
+
+// Buffered mode (acc/components/ComponentBase.py:79-107 with the launch of
+// StochasticComponentBase.call_process, :21-55): the chain runs in place over
+// an (n, 10) array, stream tid taking rows [tid*npt, min((tid+1)*npt, n)) of
+// the partition of n, streams seeded from `seed`.  A chain that is a single
+// multiple-scattering sample writes neutron i's outputs to rows
+// [i*M, i*M + num) of out_ms (n*M, 10) and leaves the rest zero.
+int64_t orc_process(const mcv_comp_desc* comps, int ncomp, const double* offsets, const double* rotmats,
+                    double* neutrons, int64_t n, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
+                    double* hist, double* out_ms, int nthreads_omp) {
+  Chain C;
+  build_chain(comps, ncomp, offsets, rotmats, flags, C);
+  if (C.ms_index >= 0 && (ncomp != 1 || !out_ms)) return -1;
+  int64_t part[3];
+  orc_partition((uint64_t)n, ntotalthreads, tpb, part);
+  int64_t nthreads = part[1], npt = part[2];
+  std::vector<Rng> states(nthreads > 0 ? nthreads : 1);
+  init_states(seed, nthreads, states.data());
+  int nt = nthreads_omp > 0 ? nthreads_omp : omp_get_max_threads();
+  std::vector<std::vector<double>> hists(nt, std::vector<double>(C.hist_total, 0.0));
+  int64_t processed = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : processed)
+  {
+    double* h = hists[omp_get_thread_num()].data();
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t tid = 0; tid < nthreads; ++tid) {
+      Rng rng = states[tid];
+      int64_t start = tid * npt;
+      int64_t end = std::min<int64_t>(start + npt, n);
+      for (int64_t idx = start; idx < end; ++idx) {
+        processed += 1;
+        double* nn = neutrons + 10 * idx;
+        if (C.ms_index < 0) {
+          run_tail(C, 0, rng, nn, h, idx);
+        } else {
+          const Sample& S = C.comps[0].sample;
+          const int M = S.max_scattered;
+          double* o = out_ms + 10 * (int64_t)M * idx;
+          std::memset(o, 0, 10 * (size_t)M * sizeof(double));
+          scatterM(rng, S, o, M, nn);
         }
       }
     }

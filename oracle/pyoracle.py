@@ -64,6 +64,10 @@ def lib():
         L.orc_component.argtypes = [ctypes.POINTER(CompDesc), _dp, _up, _dp, _dp, ctypes.c_int]
         L.orc_partition.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _ip]
         L.orc_set_trans_nudge.argtypes = [ctypes.c_int]
+        L.orc_process.restype = ctypes.c_int64
+        L.orc_process.argtypes = [ctypes.POINTER(CompDesc), ctypes.c_int, _dp, _dp, _dp, ctypes.c_int64,
+                                  ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, _dp, _dp,
+                                  ctypes.c_int]
         L.orc_run.restype = ctypes.c_int64
         L.orc_run.argtypes = [ctypes.POINTER(CompDesc), ctypes.c_int, _dp, _dp, ctypes.c_uint64, ctypes.c_uint64,
                               ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int64,
@@ -221,6 +225,36 @@ def run(descs, offsets, rotmats, N, seed, ntotalthreads=0, tpb=0, gate=True, tid
         else:
             hists.append(None)
     return hists, processed, dstates[:dc], dalive[:dc]
+
+
+def process(descs, offsets, rotmats, neutrons, seed, ntotalthreads=0, tpb=0, gate=True, nthreads=0):
+    """Buffered mode: the chain over an (n, 10) array (a copy is returned)
+    with the partition of n and streams seeded from `seed`.  Returns
+    (neutrons after, hist list, processed); a single multiple-scattering
+    sample returns its (n*M, 10) output rows instead of the neutrons."""
+    D = _Descs(descs)
+    off = np.ascontiguousarray(offsets, dtype=np.float64).reshape(-1)
+    rot = np.ascontiguousarray(rotmats, dtype=np.float64).reshape(-1)
+    arr = np.array(neutrons, dtype=np.float64, order="C").reshape(-1, 10)
+    nb = [int(d.get("nbins", 0)) for d in descs]
+    hist = np.zeros(max(3 * sum(nb), 1))
+    ms = [d for d in descs if d["type"] == 5]
+    out = None
+    if ms:
+        M = int(np.asarray(ms[0]["iparams"])[2])
+        out = np.zeros((arr.shape[0] * M, 10))
+    processed = lib().orc_process(D.arr, D.n, _d(off), _d(rot), _d(arr), arr.shape[0], int(seed),
+                                  int(ntotalthreads), int(tpb), 1 if gate else 0, _d(hist), _d(out), int(nthreads))
+    if processed < 0:
+        raise ValueError("a multiple-scattering sample is processed on its own")
+    hists, o = [], 0
+    for n in nb:
+        if n:
+            hists.append(hist[o:o + 3 * n].reshape(3, n).copy())
+            o += 3 * n
+        else:
+            hists.append(None)
+    return (out if ms else arr), hists, processed
 
 
 def run_instrument(instrument, N, seed, **kw):
