@@ -26,10 +26,14 @@ MCV_GUIDE = 2
 MCV_GUIDE_TAPERING = 3
 MCV_SAMPLE_HSS = 4
 MCV_SAMPLE_HMS = 5
+MCV_SLIT = 6
+MCV_BEAMSTOP = 7
 MCV_MON_PSD = 10
 MCV_MON_WAVELENGTH = 11
 MCV_MON_IQE = 12
 MCV_DET_TUBES = 13
+MCV_MON_PSD_4PI = 14
+MCV_MON_POSDIV = 15
 MCV_SRC_IQE_NORMALIZER = 20
 MCV_SRC_STORAGE = 21
 

@@ -3,7 +3,7 @@
 Bins (Q, E) of the scattered neutron inside an angular window; position is not
 used.  `getHistogram` divides by a Monte-Carlo normalizer (getNormalization,
 :286-344) computed on the device with a fixed seed, where the reference uses
-unseeded numpy draws.  Device routine: mcv::iqe_monitor.
+unseeded numpy draws.  Device routine: mcv::iqe_bin.
 """
 import numpy as np
 

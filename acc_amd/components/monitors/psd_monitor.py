@@ -1,7 +1,7 @@
 """PSD_monitor (acc/components/monitors/psd_monitor.py:29-96).
 
 Propagates the neutron to z=0 without writing it back, bins (x, y) strictly
-inside the window, adds N, p, p^2.  Device routine: mcv::psd_monitor.
+inside the window, adds N, p, p^2.  Device routine: mcv::psd_bin.
 """
 import math
 

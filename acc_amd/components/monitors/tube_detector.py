@@ -8,7 +8,7 @@ bins over [tof_min, tof_max) seconds.  Like the reference monitors
 (acc/components/monitors/psd_monitor.py:79-96) it does not change the neutron
 and accumulates N, sum p, sum p^2 -- here with FP64 atomics straight into HBM,
 since the (3, ntubes*npixels*ntof) histogram is larger than any cache.
-Device routine: mcv::tube_detector.
+Device routine: mcv::tube_bin.
 """
 import numpy as np
 

@@ -1,7 +1,7 @@
 """Wavelength_monitor (acc/components/monitors/wavelength_monitor.py:15-171).
 
 Propagates to z=0 without writing back, lambda = 2 pi / (v V2K), adds N, p,
-p^2.  Device routine: mcv::wavelength_monitor.
+p^2.  Device routine: mcv::wavelength_bin.
 """
 import numpy as np
 

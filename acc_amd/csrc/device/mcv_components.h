@@ -146,6 +146,46 @@ __device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P
 
 __device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) { guide_t(n, P, NoTrack{}); }
 
+// ------------------------------------------------------------------ Slit / Beamstop
+// acc/components/optics/slit.py:72-105, beamstop.py:70-103.
+// P = xmin, xmax, ymin, ymax, radius^2, cut.  A neutron that never reaches
+// z = 0 is left as it is; below the weight cut or stopped it is absorbed
+// (p = -1, the rest of the state unchanged); otherwise it moves to z = 0.
+template <bool BEAMSTOP>
+__device__ __forceinline__ void aperture(Neutron& n, const double* __restrict__ P) {
+  const double z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+  if ((z < 0 && vz <= 0) || (z > 0 && vz >= 0)) return;
+  double x = n.x, y = n.y, t = n.t;
+  if (n.p < P[5]) {
+    n.p = -1;
+    return;
+  }
+  if (z != 0) {
+    const double dt = -z / vz;
+    x += vx * dt;
+    y += vy * dt;
+    t += dt;
+  }
+  bool stop;
+  if (P[4] != 0) {
+    stop = BEAMSTOP ? (x * x + y * y <= P[4]) : (x * x + y * y > P[4]);
+  } else {
+    const bool within = P[0] <= x && x <= P[1] && P[2] <= y && y <= P[3];
+    stop = BEAMSTOP ? within : !within;
+  }
+  if (stop) {
+    n.p = -1;
+    return;
+  }
+  n.x = x;
+  n.y = y;
+  n.z = 0.;
+  n.t = t;
+}
+
+__device__ __forceinline__ void slit(Neutron& n, const double* __restrict__ P) { aperture<false>(n, P); }
+__device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ P) { aperture<true>(n, P); }
+
 // ------------------------------------------------------------------ Guide_tapering
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
@@ -303,6 +343,56 @@ __device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double
   int64_t iL = (int64_t)floor((L - Lmin) / (Lmax - Lmin) * (double)nchan);
   iL = iL < 0 ? 0 : (iL < nchan ? iL : nchan - 1);
   return iL;
+}
+
+// acc/components/monitors/posdiv_monitor.py:70-95.  P = xmin, xmax, ymin, ymax, maxdiv, npos, ndiv
+__device__ __forceinline__ int64_t posdiv_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t npos = (int64_t)P[5], ndiv = (int64_t)P[6];
+  nbins = npos * ndiv;
+  double x, y, t;
+  prop_z0(n, x, y, t);
+  if (n.t > t) return -1;
+  const double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], maxdiv = P[4];
+  if (!(x > xmin && x < xmax && y > ymin && y < ymax)) return -1;
+  const double div = atan(n.vx / n.vz) * (180. / PI);
+  if (!(div < maxdiv && div > -maxdiv)) return -1;
+  int64_t ix = (int64_t)floor((x - xmin) / (xmax - xmin) * (double)npos);
+  int64_t idiv = (int64_t)floor((div + maxdiv) / (2 * maxdiv) * (double)ndiv);
+  ix = ix < 0 ? 0 : (ix < npos ? ix : npos - 1);
+  idiv = idiv < 0 ? 0 : (idiv < ndiv ? idiv : ndiv - 1);
+  return ix * ndiv + idiv;
+}
+
+// acc/components/monitors/psd_monitor_4pi.py:59-81.  P = phimin, phimax,
+// thetamin, thetamax, radius, nphi, ntheta.  Moves the neutron onto the sphere
+// (the reference's monitor is destructive) and bins (phi, theta) with the
+// reference's index formulas (theta's rounds half up).  A neutron whose line
+// misses the sphere is not counted and left as it is.
+__device__ __forceinline__ int64_t psd4pi_bin(Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t nphi = (int64_t)P[5], ntheta = (int64_t)P[6];
+  nbins = nphi * ntheta;
+  const double R = P[4];
+  const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
+  // acc/geometry/arrow_intersect.py:155-168
+  const double r0dotv = x * vx + y * vy + z * vz;
+  const double cx = y * vz - z * vy, cy = z * vx - x * vz, cz = x * vy - y * vx;
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double r0Xv2 = cx * cx + cy * cy + cz * cz;
+  const double b2m4ac = v2 * R * R - r0Xv2;
+  if (!(b2m4ac >= 0)) return -1;
+  const double sq = sqrt(b2m4ac);
+  double t1 = -(r0dotv + sq) / v2;
+  const double t2 = -(r0dotv - sq) / v2;
+  if (t2 < 0) return -1;
+  if (t1 < 0) t1 = t2;
+  prop_dt_inplace(n, t1);
+  const double phi = atan2(n.x, n.z);
+  const double theta = asin(n.y / R);
+  int64_t iphi = (int64_t)((double)nphi * (phi / 2 / PI + 0.5));
+  int64_t itheta = (int64_t)((double)ntheta * (theta + PI / 2) / PI + 0.5);
+  iphi = iphi < 0 ? 0 : (iphi < nphi ? iphi : nphi - 1);
+  itheta = itheta < 0 ? 0 : (itheta < ntheta ? itheta : ntheta - 1);
+  return iphi * ntheta + itheta;
 }
 
 // acc/components/monitors/iqe_monitor.py:251-284.

@@ -274,6 +274,22 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
       if ((rc = expect(d, i, 10, 0))) return rc;
       copyp(10);
       return MCV_OK;
+    case MCV_SLIT:
+    case MCV_BEAMSTOP:
+      if ((rc = expect(d, i, 6, 0))) return rc;
+      copyp(6);
+      return MCV_OK;
+    case MCV_MON_PSD_4PI:
+    case MCV_MON_POSDIV: {
+      if ((rc = expect(d, i, 5, 2))) return rc;
+      if (d.iparams[0] < 1 || d.iparams[1] < 1 || d.n_bins != d.iparams[0] * d.iparams[1])
+        return fail(MCV_ERR_ARG, "component %d: n_bins != n0*n1", i);
+      copyp(5);
+      out.params.push_back((double)d.iparams[0]);
+      out.params.push_back((double)d.iparams[1]);
+      out.nbins = d.n_bins;
+      return MCV_OK;
+    }
     case MCV_GUIDE_TAPERING: {
       if ((rc = expect(d, i, 9, 1))) return rc;
       const int64_t nseg = d.iparams[0];

@@ -161,6 +161,24 @@ def _storage_psd():
     return i
 
 
+def _aperture_monitors():
+    """Source -> slit -> circular beamstop -> PosDiv monitor -> PSD_monitor_4Pi:
+    the cheap optics and monitors of the reference's sgm/sphere scripts."""
+    from .instrument import instrument
+    from .configs import _source
+    from .components.optics.slit import Slit
+    from .components.optics.beamstop import Beamstop
+    from .components.monitors.posdiv_monitor import PosDiv_monitor
+    from .components.monitors.psd_monitor_4pi import PSD_monitor_4Pi
+    i = instrument("aperture-monitors")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(Slit('slit', width=0.08, height=0.06, cut=0.0), position=(0, 0, 9.8))
+    i.append(Beamstop('stop', radius=0.01), position=(0, 0, 9.9))
+    i.append(PosDiv_monitor('posdiv', xwidth=0.1, yheight=0.1, maxdiv=0.5, npos=40, ndiv=40), position=(0, 0, 10.0))
+    i.append(PSD_monitor_4Pi('m4pi', radius=0.5, nphi=60, ntheta=30), position=(0, 0, 10.0))
+    return i
+
+
 def single(comp):
     """A one-component chain (the buffered-mode unit)."""
     from .instrument import instrument
@@ -183,6 +201,9 @@ def process_units():
         "psd": lambda: single(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=20, ny=20)),
         "hms": lambda: single(hms.components[1]),
         "storage": lambda: single(_storage_psd().components[0]),
+        "slit": lambda: single(__import__("acc_amd.components.optics.slit", fromlist=["Slit"]).Slit("s", width=1)),
+        "beamstop": lambda: single(
+            __import__("acc_amd.components.optics.beamstop", fromlist=["Beamstop"]).Beamstop("b", width=1)),
         "c2-chain": configs.c2,
     }
 
@@ -206,6 +227,7 @@ def instruments():
         "c5": configs.c5,
         "grid-sphere": grid_sphere,
         "storage-psd": _storage_psd,
+        "aperture-monitors": _aperture_monitors,
     }
 
 

@@ -79,10 +79,16 @@ extern "C" {
 #define MCV_GUIDE_TAPERING 3
 #define MCV_SAMPLE_HSS 4
 #define MCV_SAMPLE_HMS 5
+#define MCV_SLIT 6      /* params[6] = xmin, xmax, ymin, ymax, radius^2, cut (acc/components/optics/slit.py:58-60) */
+#define MCV_BEAMSTOP 7  /* params[6] as MCV_SLIT (acc/components/optics/beamstop.py:58-60) */
 #define MCV_MON_PSD 10
 #define MCV_MON_WAVELENGTH 11
 #define MCV_MON_IQE 12
 #define MCV_DET_TUBES 13
+#define MCV_MON_PSD_4PI 14 /* params[5] = phimin, phimax, thetamin, thetamax, radius; iparams = nphi, ntheta
+                              (acc/components/monitors/psd_monitor_4pi.py:47-50) */
+#define MCV_MON_POSDIV 15  /* params[5] = xmin, xmax, ymin, ymax, maxdiv; iparams = npos, ndiv
+                              (acc/components/monitors/posdiv_monitor.py:53-56) */
 /* engine-internal sources */
 #define MCV_SRC_IQE_NORMALIZER 20 /* params[3] = Ei, Emin, Emax: the I(Q,E) monitor's
                                      Monte-Carlo normalizer neutrons

@@ -2,7 +2,7 @@
 // build of the oracle (oracle_flops.cpp).  TEST/MEASUREMENT INFRASTRUCTURE.
 //
 // Counting convention: +, -, *, / and sqrt = 1 op each; each transcendental
-// (exp, sin, cos, tanh, atan, atan2) = one "trans" event (weighted 20 by the
+// (exp, sin, cos, tanh, atan, atan2, asin, acos) = one "trans" event (weighted 20 by the
 // reader); negation, comparisons, floor, fabs and conversions are free.  A
 // region marked ORC_PAUSE/ORC_RESUME (loop-invariant work) is not counted.
 #pragma once
@@ -65,6 +65,7 @@ inline orcflops::CReal tanh(orcflops::CReal a) { return orcflops::c_trans1(std::
 inline orcflops::CReal atan(orcflops::CReal a) { return orcflops::c_trans1(std::atan(a.v)); }
 inline orcflops::CReal atan2(orcflops::CReal a, orcflops::CReal b) { return orcflops::c_trans1(std::atan2(a.v, b.v)); }
 inline orcflops::CReal acos(orcflops::CReal a) { return orcflops::c_trans1(std::acos(a.v)); }
+inline orcflops::CReal asin(orcflops::CReal a) { return orcflops::c_trans1(std::asin(a.v)); }
 inline orcflops::CReal floor(orcflops::CReal a) { return orcflops::CReal(std::floor(a.v)); }
 inline orcflops::CReal fabs(orcflops::CReal a) { return orcflops::CReal(std::fabs(a.v)); }
 inline bool isnan(orcflops::CReal a) { return std::isnan(a.v); }

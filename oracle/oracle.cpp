@@ -75,6 +75,7 @@ static inline double tr(double v) {
 #define TR_TANH(x) tr(std::tanh(x))
 #define TR_ATAN(x) tr(std::atan(x))
 #define TR_ATAN2(a, b) tr(std::atan2(a, b))
+#define TR_ASIN(x) tr(std::asin(x))
 
 // mcni.utils.conversion (McStas constants, not vendored)
 static const double V2K = 1.58825361e-3;
@@ -1289,6 +1290,41 @@ static int scatterM(Rng& rng, const Sample& S, double* out, int out_len, double*
   return out_index;
 }
 
+// ---------------------------------------------------------------- slit, beamstop
+// acc/components/optics/slit.py:72-105 and beamstop.py:70-103 (absorb = p -> -1)
+static void aperture(double* n, const double* P, bool beamstop) {
+  double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], radius_squared = P[4], cut = P[5];
+  double x = n[0], y = n[1], z = n[2], vx = n[3], vy = n[4], vz = n[5];
+  if ((z < 0 && vz <= 0) || (z > 0 && vz >= 0)) return;
+  double t = n[8], prob = n[9];
+  if (prob < cut) {
+    n[9] = -1;
+    return;
+  }
+  if (z != 0) {
+    double dt = -z / vz;
+    x += vx * dt;
+    y += vy * dt;
+    t += dt;
+  }
+  if (radius_squared != 0) {
+    if (beamstop ? (x * x + y * y <= radius_squared) : (x * x + y * y > radius_squared)) {
+      n[9] = -1;
+      return;
+    }
+  } else {
+    bool within = xmin <= x && x <= xmax && ymin <= y && y <= ymax;
+    if (beamstop ? within : !within) {
+      n[9] = -1;
+      return;
+    }
+  }
+  n[0] = x;
+  n[1] = y;
+  n[2] = 0.;
+  n[8] = t;
+}
+
 // ---------------------------------------------------------------- monitors (R14-R17)
 // acc/components/monitors/psd_monitor.py:79-96
 static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny, double* out) {
@@ -1304,6 +1340,58 @@ static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny
   if (ix >= nx) ix = nx - 1;  // the reference writes past the row here (no bounds check)
   if (iy >= ny) iy = ny - 1;
   int64_t nb = nx * ny, b = ix * ny + iy;
+  out[b] += 1;
+  out[nb + b] += p;
+  out[2 * nb + b] += p * p;
+}
+
+// acc/components/monitors/posdiv_monitor.py:70-95 (NaN rejected and indices
+// clamped, as for the PSD monitor)
+static void posdiv_monitor(const double* n, const double* P, int64_t npos, int64_t ndiv, double* out) {
+  double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], maxdiv = P[4];
+  double t0 = n[8], x, y, t;
+  prop_z0(n, x, y, t);
+  if (t0 > t) return;
+  double p = n[9];
+  double vx = n[3], vz = n[5];
+  if (!(x > xmin && x < xmax && y > ymin && y < ymax)) return;
+  double div = TR_ATAN(vx / vz) * (180. / PI);
+  if (!(div < maxdiv && div > -maxdiv)) return;
+  int64_t ix = (int64_t)std::floor((x - xmin) / (xmax - xmin) * npos);
+  int64_t idiv = (int64_t)std::floor((div + maxdiv) / (2 * maxdiv) * ndiv);
+  ix = std::min<int64_t>(std::max<int64_t>(ix, 0), npos - 1);
+  idiv = std::min<int64_t>(std::max<int64_t>(idiv, 0), ndiv - 1);
+  int64_t nb = npos * ndiv, b = ix * ndiv + idiv;
+  out[b] += 1;
+  out[nb + b] += p;
+  out[2 * nb + b] += p * p;
+}
+
+// acc/components/monitors/psd_monitor_4pi.py:59-81 with
+// acc/geometry/arrow_intersect.py:155-168 (moves the neutron onto the sphere;
+// a line that misses the sphere is not counted here)
+static void psd4pi_monitor(double* n, const double* P, int64_t nphi, int64_t ntheta, double* out) {
+  double radius = P[4];
+  double x = n[0], y = n[1], z = n[2], vx = n[3], vy = n[4], vz = n[5];
+  double r0dotv = x * vx + y * vy + z * vz;
+  double r0Xv_x = y * vz - z * vy, r0Xv_y = z * vx - x * vz, r0Xv_z = x * vy - y * vx;
+  double v2 = vx * vx + vy * vy + vz * vz;
+  double r0Xv2 = r0Xv_x * r0Xv_x + r0Xv_y * r0Xv_y + r0Xv_z * r0Xv_z;
+  double b2m4ac = v2 * radius * radius - r0Xv2;
+  if (!(b2m4ac >= 0)) return;
+  double sq = std::sqrt(b2m4ac);
+  double t1 = -(r0dotv + sq) / v2, t2 = -(r0dotv - sq) / v2;
+  if (t2 < 0) return;
+  if (t1 < 0) t1 = t2;
+  prop_dt_inplace(n, t1);
+  double phi = TR_ATAN2(n[0], n[2]);
+  double theta = TR_ASIN(n[1] / radius);
+  int64_t iphi = (int64_t)(nphi * (phi / 2 / PI + 0.5));
+  int64_t itheta = (int64_t)(ntheta * (theta + PI / 2) / PI + 0.5);
+  iphi = std::min<int64_t>(std::max<int64_t>(iphi, 0), nphi - 1);
+  itheta = std::min<int64_t>(std::max<int64_t>(itheta, 0), ntheta - 1);
+  double p = n[9];
+  int64_t nb = nphi * ntheta, b = iphi * ntheta + itheta;
   out[b] += 1;
   out[nb + b] += p;
   out[2 * nb + b] += p * p;
@@ -1444,7 +1532,11 @@ static void run_component(const Chain& C, int i, Rng& rng, double* n, double* hi
     case MCV_SOURCE_SIMPLE: source_simple(rng, n, c.P); break;
     case MCV_SRC_IQE_NORMALIZER: iqe_normalizer_source(rng, n, c.P); break;
     case MCV_GUIDE: guide(n, c.P); break;
+    case MCV_SLIT: aperture(n, c.P, false); break;
+    case MCV_BEAMSTOP: aperture(n, c.P, true); break;
     case MCV_GUIDE_TAPERING: guide_tapering(n, c.P, c.IP[0], c.data); break;
+    case MCV_MON_POSDIV: posdiv_monitor(n, c.P, c.IP[0], c.IP[1], hist + c.hist_off); break;
+    case MCV_MON_PSD_4PI: psd4pi_monitor(n, c.P, c.IP[0], c.IP[1], hist + c.hist_off); break;
     case MCV_SAMPLE_HSS: hss_propagate(rng, n, c.sample); break;
     case MCV_MON_PSD: psd_monitor(n, c.P, c.IP[0], c.IP[1], hist + c.hist_off); break;
     case MCV_MON_WAVELENGTH: wavelength_monitor(n, c.P, c.IP[0], hist + c.hist_off); break;
@@ -1620,7 +1712,11 @@ int orc_component(const mcv_comp_desc* d, double* n, uint64_t* state2, double* h
   switch (d->type) {
     case MCV_SOURCE_SIMPLE: source_simple(rng, n, (const double*)d->params); break;
     case MCV_GUIDE: guide(n, (const double*)d->params); break;
+    case MCV_SLIT: aperture(n, (const double*)d->params, false); break;
+    case MCV_BEAMSTOP: aperture(n, (const double*)d->params, true); break;
     case MCV_GUIDE_TAPERING: guide_tapering(n, (const double*)d->params, d->iparams[0], (const double*)d->data); break;
+    case MCV_MON_POSDIV: posdiv_monitor(n, (const double*)d->params, d->iparams[0], d->iparams[1], hist); break;
+    case MCV_MON_PSD_4PI: psd4pi_monitor(n, (const double*)d->params, d->iparams[0], d->iparams[1], hist); break;
     case MCV_SAMPLE_HSS: {
       Sample S = make_sample(*d);
       hss_propagate(rng, n, S);

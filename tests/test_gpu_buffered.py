@@ -133,3 +133,18 @@ def test_storage_chain_run(tmp_path):
     assert st["processed"] == 1000
     I = psd.out[1]
     assert I.sum() == 1000 and I[2, 2] == 1000
+
+
+def test_slit_beamstop_known_answers():
+    """The reference's exhaustive slit/beamstop known answers
+    (tests/components/optics/test_slit_beamstop.py:91-140) through
+    component.process on the GPU."""
+    from slit_kat import cases, neutrons
+    from acc_amd.components.optics.slit import Slit
+    from acc_amd.components.optics.beamstop import Beamstop
+    for name, kw, invert, passes in cases():
+        comp = (Beamstop if invert else Slit)("c", **kw)
+        start, expected = neutrons(passes)
+        got = comp.process(start)
+        assert got.shape == expected.shape, name
+        assert np.allclose(got, expected, atol=1e-15), name

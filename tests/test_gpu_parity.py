@@ -49,6 +49,8 @@ CASES = [
     ("c5", True, 150000, 4096),
     ("grid-sphere", True, 150000, 4096),
     ("storage-psd", True, 5000, 512),
+    ("aperture-monitors", True, 300000, 8192),
+    ("aperture-monitors", False, 100000, 4096),
     ("hollow-cylinder-hss", True, 300000, 8192),
     ("verdi-taper", True, 60000, 2048),
     ("c3s-small", True, 300000, 8192),

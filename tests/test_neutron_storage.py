@@ -85,3 +85,21 @@ def test_oracle_storage_source_replays_rows(golden_dir):
                                          np.zeros((7, 10)), seed=1)
     assert processed == 7
     np.testing.assert_array_equal(out, np.tile(np.arange(10.), (7, 1)))
+
+
+def test_oracle_slit_beamstop_known_answers():
+    """The reference's slit/beamstop known answers through the oracle's
+    buffered mode (the same check runs on the GPU in test_gpu_buffered.py)."""
+    import pyoracle
+    from slit_kat import cases, neutrons
+    from acc_amd.components.optics.slit import Slit
+    from acc_amd.components.optics.beamstop import Beamstop
+    for name, kw, invert, passes in cases():
+        comp = (Beamstop if invert else Slit)("c", **kw)
+        start, expected = neutrons(passes)
+        out, _, processed = pyoracle.process([comp.mcv_desc()], np.zeros((0, 3)), np.zeros((0, 3, 3)), start,
+                                             seed=1)
+        assert processed == len(start)
+        got = out[out[:, 9] > 0]
+        assert got.shape == expected.shape, name
+        assert np.allclose(got, expected, atol=1e-15), name

@@ -221,3 +221,58 @@ def test_sqe_kinematic_closure_and_weight():
         w = 2 * R * sigma * (vf / vi) * Q * (Qmax - Qmin) * (Emax - Emin) / (2 * ki * kf)
         assert n[9] == pytest.approx(w, rel=1e-6)
     assert n_ok > 300
+
+
+def _one(desc, n):
+    import pyoracle
+    out, hists, processed = pyoracle.process([desc], np.zeros((0, 3)), np.zeros((0, 3, 3)), np.array([n]), seed=1)
+    return out[0], hists[0]
+
+
+def test_posdiv_monitor_closed_form():
+    """A neutron crossing z = 0 at x with vx/vz = tan(div): bin (floor of the
+    position fraction, floor of the divergence fraction); non-destructive."""
+    from acc_amd.components.monitors.posdiv_monitor import PosDiv_monitor
+    m = PosDiv_monitor('pd', xwidth=0.2, yheight=0.2, maxdiv=2., npos=20, ndiv=40)
+    d = m.mcv_desc()
+    for x, div in [(0.013, 0.37), (-0.091, -1.93), (0.0999, 1.2), (0.004, -0.01)]:
+        vz = 1000.0
+        vx = vz * np.tan(np.radians(div))
+        n = np.array([x - vx * 1e-3, 0.01, -1.0, vx, 0, vz, 0, 0, 0, 0.5])
+        out, h = _one(d, n)
+        np.testing.assert_array_equal(out, n)
+        xs = n[0] + vx * (1.0 / vz)
+        ix = int(np.floor((xs + 0.1) / 0.2 * 20))
+        idiv = int(np.floor((np.degrees(np.arctan(vx / vz)) + 2) / 4 * 40))
+        b = ix * 40 + idiv
+        assert h[0, b] == 1 and h[1, b] == 0.5 and h[2, b] == 0.25 and h[0].sum() == 1
+    out, h = _one(d, np.array([0, 0, -1.0, 1000 * np.tan(np.radians(2.5)), 0, 1000, 0, 0, 0, 1]))
+    assert h[0].sum() == 0  # outside +-maxdiv
+
+
+def test_psd4pi_monitor_closed_form():
+    """A neutron leaving the origin in direction (phi, theta) reaches the sphere
+    at radius R after R / |v|; bins follow the reference's formulas
+    (psd_monitor_4pi.py:70-75, theta rounding half up)."""
+    from acc_amd.components.monitors.psd_monitor_4pi import PSD_monitor_4Pi
+    R = 1.5
+    m = PSD_monitor_4Pi('m4', radius=R, nphi=36, ntheta=18)
+    d = m.mcv_desc()
+    for phi, theta in [(0.3, 0.1), (-2.9, -1.2), (3.0, 1.4), (-0.01, 0.0)]:
+        u = np.array([np.cos(theta) * np.sin(phi), np.sin(theta), np.cos(theta) * np.cos(phi)])
+        v = 800.0 * u
+        n = np.array([0, 0, 0, v[0], v[1], v[2], 0, 0, 0.01, 2.0])
+        out, h = _one(d, n)
+        np.testing.assert_allclose(out[:3], R * u, rtol=1e-14, atol=1e-15)
+        assert abs(out[8] - (0.01 + R / 800.0)) < 1e-15
+        ph = np.arctan2(out[0], out[2])
+        th = np.arcsin(out[1] / R)
+        iphi = min(max(int(36 * (ph / 2 / np.pi + 0.5)), 0), 35)
+        ith = min(max(int(18 * (th + np.pi / 2) / np.pi + 0.5), 0), 17)
+        b = iphi * 18 + ith
+        assert h[0, b] == 1 and h[1, b] == 2.0 and h[0].sum() == 1
+    # a line that misses the sphere is not counted and leaves the neutron as it was
+    n = np.array([0, 5.0, -3.0, 0, 0, 100.0, 0, 0, 0, 1])
+    out, h = _one(d, n)
+    np.testing.assert_array_equal(out, n)
+    assert h[0].sum() == 0
