@@ -190,6 +190,13 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
 // D = 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf).
+// The reference's two nested loops (segments, then bounces inside a segment)
+// run here as one event loop: each pass computes the next wall or segment-end
+// time of the current segment, then either reflects or moves on to the next
+// segment.  Every neutron still performs exactly the reference's operations in
+// the reference's order, but lanes of a wavefront no longer wait for each
+// other at every segment: a wave runs max(segments + bounces) passes instead
+// of the sum over segments of the per-segment maximum.
 template <class T>
 __device__ __forceinline__ void guide_tapering_t(Neutron& n, const double* __restrict__ P,
                                                  const double* __restrict__ D, T tr) {
@@ -207,26 +214,42 @@ __device__ __forceinline__ void guide_tapering_t(Neutron& n, const double* __res
   y += vy * dt;
   z = 0.0;
   t += dt;
-  for (int seg = 0; seg < nseg; ++seg) {
-    const double zr = (double)seg * l_seg;
-    const double w1s = w1[seg], hhs = hhalf[seg], wws = wwa[seg], hhh = hha[seg], whs = whalf[seg];
-    dt = (zr - z) / vz;
-    x += vx * dt;
-    y += vy * dt;
-    z += vz * dt;
-    if (x <= -0.5 * w1s || x >= 0.5 * w1s || y <= -hhs || y >= hhs) {
-      n.p = 0;
-      return;
+  int seg = 0, nb = 0;
+  double zr = 0, w1s = 0, hhs = 0, wws = 0, hhh = 0, whs = 0, hadj = 0;
+  bool enter = nseg > 0;
+  while (true) {
+    if (enter) {  // segment entrance: guide_tapering.py:170-190
+      zr = (double)seg * l_seg;
+      w1s = w1[seg];
+      hhs = hhalf[seg];
+      wws = wwa[seg];
+      hhh = hha[seg];
+      whs = whalf[seg];
+      dt = (zr - z) / vz;
+      x += vx * dt;
+      y += vy * dt;
+      z += vz * dt;
+      if (x <= -0.5 * w1s || x >= 0.5 * w1s || y <= -hhs || y >= hhs) {
+        n.p = 0;
+        return;
+      }
+      x += 0.5 * w1s;
+      const double edge = floor(x / w1s) * w1s;
+      if (x - edge > w1s) {
+        n.p = 0;
+        return;
+      }
+      x -= (edge + 0.5 * w1s);
+      hadj = edge + (0.5 * w1s) - 0.5 * w1s;
+      nb = 0;
+      enter = false;
+    } else if (seg >= nseg) {
+      break;
     }
-    x += 0.5 * w1s;
-    const double edge = floor(x / w1s) * w1s;
-    if (x - edge > w1s) {
-      n.p = 0;
-      return;
-    }
-    x -= (edge + 0.5 * w1s);
-    const double hadj = edge + (0.5 * w1s) - 0.5 * w1s;
-    for (int nb = 0; nb < 100000; ++nb) {
+    // one pass of the bounce loop (guide_tapering.py:193-284); nb counts the
+    // reference's max_bounces passes per segment
+    bool seg_end = nb >= 100000;
+    if (!seg_end) {
       const double av = l_seg * vx, bv = wws * vz, ah = l_seg * vy, bh = hhh * vz;
       const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
       const double cv1 = -whs * l_seg - (z - zr) * wws, cv2 = x * l_seg;
@@ -249,44 +272,53 @@ __device__ __forceinline__ void guide_tapering_t(Neutron& n, const double* __res
         const double t2 = (ch1 + ch2) / vdotn_h2;
         if (t2 < t1) { t1 = t2; i = 4; }
       }
-      if (i == 0) break;
-      x += vx * t1;
-      y += vy * t1;
-      z += vz * t1;
-      t += t1;
-      const bool vert = i <= 2;
-      const double wh = vert ? wws : hhh;
-      const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
-      const double nlen2 = l_seg * l_seg + wh * wh;
-      const double q = V2K * (-2.0) * vdn / sqrt(nlen2);
-      const double d = 2.0 * vdn / nlen2;
-      const double dl = d * l_seg;
-      if (i == 1) vx = vx - dl;
-      else if (i == 2) vx = vx + dl;
-      else if (i == 3) vy = vy - dl;
-      else vy = vy + dl;
-      vz = vz - d * wh;
-      const double mxy = vert ? P[6] : P[7];
-      if (mxy == 0) {
-        x += hadj;
-        break;
-      }
-      double m = mxy, dm = 1.0;
-      if (m <= 0.0) {
-        const double ratio = vert ? w1[0] / w1s : h1[0] / h1[seg];
-        m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
-        dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
-      }
-      const double R = calc_reflectivity(q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W);
-      if (R > 1e-10) {
-        tr.mirror(vert ? 0 : 1, q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W, dm);
-        prob *= R;
+      if (i == 0) {
+        seg_end = true;  // the neutron leaves the segment
       } else {
-        n.p = 0;
-        return;
+        x += vx * t1;
+        y += vy * t1;
+        z += vz * t1;
+        t += t1;
+        const bool vert = i <= 2;
+        const double wh = vert ? wws : hhh;
+        const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
+        const double nlen2 = l_seg * l_seg + wh * wh;
+        const double q = V2K * (-2.0) * vdn / sqrt(nlen2);
+        const double d = 2.0 * vdn / nlen2;
+        const double dl = d * l_seg;
+        if (i == 1) vx = vx - dl;
+        else if (i == 2) vx = vx + dl;
+        else if (i == 3) vy = vy - dl;
+        else vy = vy + dl;
+        vz = vz - d * wh;
+        const double mxy = vert ? P[6] : P[7];
+        if (mxy == 0) {
+          x += hadj;  // the reference adds hadj here and again at the segment end
+          seg_end = true;
+        } else {
+          double m = mxy, dm = 1.0;
+          if (m <= 0.0) {
+            const double ratio = vert ? w1[0] / w1s : h1[0] / h1[seg];
+            m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
+            dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
+          }
+          const double R = calc_reflectivity(q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W);
+          if (R > 1e-10) {
+            tr.mirror(vert ? 0 : 1, q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W, dm);
+            prob *= R;
+          } else {
+            n.p = 0;
+            return;
+          }
+          ++nb;
+        }
       }
     }
-    x += hadj;
+    if (seg_end) {
+      x += hadj;
+      ++seg;
+      enter = seg < nseg;
+    }
   }
   n.x = x; n.y = y; n.z = z; n.vx = vx; n.vy = vy; n.vz = vz;
   n.t = t;

@@ -82,12 +82,17 @@ __device__ __forceinline__ void lds_hist_zero(const LaunchArgs& A) {
 __device__ __forceinline__ void lds_hist_flush(const LaunchArgs& A) {
   if (A.lds_nseg == 0) return;
   __syncthreads();
-  for (int k = 0; k < A.lds_nseg; ++k) {
-    const double* L = mcv_lds_hist + A.lds_loff[k];
-    double* G = A.H + A.lds_hoff[k];
-    for (int j = threadIdx.x; j < A.lds_len[k]; j += blockDim.x) {
-      const double v = L[j];
-      if (v != 0) unsafeAtomicAdd(G + j, v);
+  // constant indices only: the batch kernel keeps its LaunchArgs copy out of
+  // scratch memory only if no field is indexed at run time
+#pragma unroll
+  for (int k = 0; k < MCV_LDS_SEG; ++k) {
+    if (k < A.lds_nseg) {
+      const double* L = mcv_lds_hist + A.lds_loff[k];
+      double* G = A.H + A.lds_hoff[k];
+      for (int j = threadIdx.x; j < A.lds_len[k]; j += blockDim.x) {
+        const double v = L[j];
+        if (v != 0) unsafeAtomicAdd(G + j, v);
+      }
     }
   }
   __syncthreads();
