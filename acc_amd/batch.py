@@ -26,11 +26,12 @@ from .instrument import calcTransformations
 class BatchedScan:
     """Instances of one chain structure, run together on the GPU."""
 
-    def __init__(self, instruments, compact=True, block=None, waves_per_eu=None):
+    def __init__(self, instruments, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
         from .fuser import generate_source, compile_source
         if not instruments:
             raise ValueError("empty scan")
-        srcs = [generate_source(instr, compact=compact, block=block, waves_per_eu=waves_per_eu)[0]
+        srcs = [generate_source(instr, compact=compact, block=block, waves_per_eu=waves_per_eu,
+                                streams_per_lane=streams_per_lane)[0]
                 for instr in instruments]
         for k, s in enumerate(srcs):
             if s != srcs[0]:
@@ -38,6 +39,7 @@ class BatchedScan:
         self.code_object, self.kernel_name = compile_source(srcs[0])
         self.block = int(block or config.kernel_block)
         self.waves_per_eu = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
+        self.streams_per_lane = int(streams_per_lane or config.kernel_streams_per_lane)
         self.instruments = list(instruments)
         descs, offs, rots = [], [], []
         for instr in self.instruments:

@@ -16,6 +16,8 @@ threads_per_block = 512
 kernel_block = 256
 # occupancy target (waves per SIMD) of the fused kernels; None = compiler's choice
 kernel_waves_per_eu = None
+# random streams per lane in compacted kernels whose later stages draw
+kernel_streams_per_lane = 4
 floattype = "float64"
 rng_seed = 1
 gate_dead_neutrons = True

@@ -31,7 +31,15 @@
 #endif
 #define MCV_QCAP (2 * MCV_BLOCK)  // >= (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
 
+// Random streams per lane (1 except in stream-carrying compacted kernels): a
+// workgroup serves MCV_BLOCK * MCV_SPL consecutive streams, lane l owning
+// streams l, l + MCV_BLOCK, ...  The runtime reads it from `mcv_streams_per_lane`.
+#ifndef MCV_SPL
+#define MCV_SPL 1
+#endif
+
 extern "C" __device__ int mcv_block_size = MCV_BLOCK;
+extern "C" __device__ int mcv_streams_per_lane = MCV_SPL;
 
 // Optional occupancy target (waves per SIMD) the register allocator must meet;
 // above the natural occupancy of a chain it trades spills for latency hiding.
@@ -208,13 +216,30 @@ struct QueueCore {
 // deterministic phase B: up to (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
 struct QueueLDS : QueueCore<MCV_QCAP> {};
 
-// drawing phase B: at most one queued neutron per lane, with its stream state
-struct QueueRngLDS : QueueCore<MCV_BLOCK> {
-  uint64_t s0[MCV_BLOCK], s1[MCV_BLOCK];  // stream state carried with the neutron
-  int owner[MCV_BLOCK];
-  uint64_t r0[MCV_BLOCK], r1[MCV_BLOCK];  // stream state handed back to its owner lane
-  int returned[MCV_BLOCK];
+// drawing phase B: the states and next neutron indices of all the
+// workgroup's streams live here.  A queued neutron names its stream slot;
+// that stream is paused while the neutron waits, so its slot holds the
+// neutron's stream state and its index is the slot's next index - 1.
+struct QueueRngLDS {
+  double x[MCV_QCAP], y[MCV_QCAP], z[MCV_QCAP], vx[MCV_QCAP], vy[MCV_QCAP], vz[MCV_QCAP], t[MCV_QCAP],
+      p[MCV_QCAP];
+  int owner[MCV_QCAP];  // stream slot: lane * MCV_SPL + stream of the lane
+  uint64_t st0[MCV_BLOCK * MCV_SPL], st1[MCV_BLOCK * MCV_SPL];
+  int64_t next[MCV_BLOCK * MCV_SPL];
+  int returned[MCV_BLOCK * MCV_SPL];
+  int count;  // slot allocator
+  int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
 };
+
+__device__ __forceinline__ void q_store_state(QueueRngLDS& q, int s, const Neutron& n) {
+  q.x[s] = n.x; q.y[s] = n.y; q.z[s] = n.z; q.vx[s] = n.vx; q.vy[s] = n.vy; q.vz[s] = n.vz;
+  q.t[s] = n.t; q.p[s] = n.p;
+}
+
+__device__ __forceinline__ void q_load_state(const QueueRngLDS& q, int s, Neutron& n) {
+  n.x = q.x[s]; n.y = q.y[s]; n.z = q.z[s]; n.vx = q.vx[s]; n.vy = q.vy[s]; n.vz = q.vz[s];
+  n.t = q.t[s]; n.p = q.p[s]; n.s1 = 0; n.s2 = 0;
+}
 
 template <class Q>
 __device__ __forceinline__ void q_store(Q& q, int s, const Neutron& n, int64_t idx) {
@@ -316,75 +341,103 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
 }
 
 // Phase B draws too: a surviving neutron carries its stream's state into the
-// queue and its owner lane pauses that stream until phase B hands the state
-// back, so every stream still draws for its neutrons one after another in
-// the reference's order.  Phase B runs over whole wavefronts of queued
-// neutrons (leftovers wait for the next flush) once half a workgroup is
-// queued, or over everything when no lane can run phase A.
+// queue and that stream pauses until phase B hands the state back, so every
+// stream still draws for its neutrons one after another in the reference's
+// order.  Each lane owns MCV_SPL streams and produces from any of them that is
+// not paused, so lanes keep producing while their earlier neutrons wait.
+// Phase B runs over a full workgroup of queued neutrons, or over everything
+// queued when no lane can produce.  `tid` is the lane's first stream.
 template <class Chain>
 __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q) {
   const int lt = threadIdx.x;
+  constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
   lds_hist_zero(A);
   if (lt == 0) q.count = 0;
-  q.returned[lt] = 0;
-  Rng rng;
-  int64_t idx, end;
-  stream_range(A, tid, rng, idx, end);
-  const int64_t mine = end - idx;
-  int par = 0, cnt = 0;
-  bool busy = false;
+  unsigned busy = 0, done = 0;  // per stream of this lane: paused / no neutrons left
+  int64_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < MCV_SPL; ++k) {
+    Rng r;
+    int64_t idx, end;
+    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
+    const int o = lt * MCV_SPL + k;
+    q.st0[o] = r.s0;
+    q.st1[o] = r.s1;
+    q.next[o] = idx;
+    q.returned[o] = 0;
+    mine += end - idx;
+    if (idx >= end) done |= 1u << k;
+  }
+  int par = 0, cnt = 0, rr = 0;
   __syncthreads();
   while (true) {
+    // a stream that is neither paused nor exhausted, round robin from rr
+    const unsigned avail = ~(busy | done) & ALL;
     bool has = false;
     Neutron n;
     n.s1 = 0;
     n.s2 = 0;
-    const int64_t my = idx;
-    if (!busy && idx < end) {
-      has = Chain::phase_a(rng, n, A, my) != 0;
+    Rng w;
+    int64_t my = 0;
+    int sel = -1;
+    if (avail) {
+      const unsigned rot = ((avail >> rr) | (avail << (MCV_SPL - rr))) & ALL;
+      sel = (rr + __builtin_ctz(rot)) % MCV_SPL;
+      const int o = lt * MCV_SPL + sel;
+      w.s0 = q.st0[o];
+      w.s1 = q.st1[o];
+      my = q.next[o];
+      has = Chain::phase_a(w, n, A, my) != 0;
       if (!has) dump_neutron(A, my, n, 0);
-      ++idx;
+      q.st0[o] = w.s0;
+      q.st1[o] = w.s1;
+      q.next[o] = my + 1;
+      int64_t s_idx, s_end;
+      Rng unused;
+      stream_range(A, tid + (int64_t)sel * MCV_BLOCK, unused, s_idx, s_end);
+      if (my + 1 >= s_end) done |= 1u << sel;
+      if (has) busy |= 1u << sel;
+      rr = (sel + 1) % MCV_SPL;
     }
     const int s = q_append(q, has);
     if (has) {
-      q_store(q, s, n, my);
-      q.s0[s] = rng.s0;
-      q.s1[s] = rng.s1;
-      q.owner[s] = lt;
-      busy = true;
+      q_store_state(q, s, n);
+      q.owner[s] = lt * MCV_SPL + sel;
     }
     int n_new;
     bool can_a;
-    round_totals(q, par, has, !busy && idx < end, n_new, can_a);
-    cnt += n_new;  // <= MCV_BLOCK: one queued neutron per busy lane
-    if (cnt >= MCV_BLOCK / 2 || (!can_a && cnt > 0)) {
-      const int take = can_a ? (cnt & ~63) : cnt;
+    round_totals(q, par, has, (~(busy | done) & ALL) != 0, n_new, can_a);
+    cnt += n_new;  // < 2 * MCV_BLOCK: a full workgroup is taken as soon as one is queued
+    if (cnt >= MCV_BLOCK || (!can_a && cnt > 0)) {
+      const int take = cnt >= MCV_BLOCK ? MCV_BLOCK : cnt;
       if (lt < take) {
         Neutron b;
         const int slot = cnt - take + lt;
-        const int64_t bidx = q_load(q, slot, b);
-        Rng r;
-        r.s0 = q.s0[slot];
-        r.s1 = q.s1[slot];
-        const int alive = (int)Chain::phase_b(r, b, A);
-        dump_neutron(A, bidx, b, alive);
+        q_load_state(q, slot, b);
         const int o = q.owner[slot];
-        q.r0[o] = r.s0;
-        q.r1[o] = r.s1;
+        Rng r;
+        r.s0 = q.st0[o];
+        r.s1 = q.st1[o];
+        const int alive = (int)Chain::phase_b(r, b, A);
+        dump_neutron(A, q.next[o] - 1, b, alive);
+        q.st0[o] = r.s0;  // the stream continues from here
+        q.st1[o] = r.s1;
         q.returned[o] = 1;
       }
       cnt -= take;
       __syncthreads();
-      if (q.returned[lt]) {
-        rng.s0 = q.r0[lt];
-        rng.s1 = q.r1[lt];
-        q.returned[lt] = 0;
-        busy = false;
+#pragma unroll
+      for (int k = 0; k < MCV_SPL; ++k) {
+        const int o = lt * MCV_SPL + k;
+        if (q.returned[o]) {
+          q.returned[o] = 0;
+          busy &= ~(1u << k);
+        }
       }
       if (lt == 0) q.count = cnt;
       __syncthreads();
     } else if (!can_a) {
-      break;  // nothing queued and no lane has work
+      break;  // nothing queued and no stream has neutrons left
     }
   }
   count_processed(A.counter, mine);
@@ -416,7 +469,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   A.counter = B.inst_processed + lo;
   A.dump = nullptr;
   A.dump_count = 0;
-  tid = (item - B.item0[lo]) * MCV_BLOCK + threadIdx.x;
+  tid = (item - B.item0[lo]) * (MCV_BLOCK * MCV_SPL) + threadIdx.x;
   return true;
 }
 
@@ -449,7 +502,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 #define MCV_DEFINE_KERNELS_COMPACTED_RNG(NAME, CHAIN)                                          \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     __shared__ mcv::QueueRngLDS q;                                                            \
-    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
+    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x; \
     mcv::run_block_compacted_rng<CHAIN>(A, tid, q);                                           \
   }                                                                                           \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \

@@ -438,6 +438,7 @@ __global__ void mcv_zero_f64(double* p, int64_t n) {
 struct mcv_instr {
   int ncomp = 0;
   int block = 256;  // workgroup size of the code object's kernels
+  int spl = 1;      // random streams per lane (a workgroup serves block * spl streams)
   std::vector<int> types;
   std::vector<int> nparams;  // reference-order params per component (settable)
   mcv::LaunchArgs args{};
@@ -481,18 +482,32 @@ namespace {
 
 // Workgroup size a chain code object was built for (its `mcv_block_size`
 // global; 256 for code objects without one).
-int module_block(hipModule_t mod, int* block) {
-  *block = 256;
+int module_int(hipModule_t mod, const char* name, int dflt, int* out) {
+  *out = dflt;
   hipDeviceptr_t p = nullptr;
   size_t sz = 0;
-  if (hipModuleGetGlobal(&p, &sz, mod, "mcv_block_size") != hipSuccess) {
+  if (hipModuleGetGlobal(&p, &sz, mod, name) != hipSuccess) {
     (void)hipGetLastError();
     return MCV_OK;
   }
-  if (sz != sizeof(int)) return fail(MCV_ERR_MODULE, "mcv_block_size has size %zu", sz);
-  HIPCHECK(hipMemcpyDtoH(block, p, sizeof(int)));
+  if (sz != sizeof(int)) return fail(MCV_ERR_MODULE, "%s has size %zu", name, sz);
+  HIPCHECK(hipMemcpyDtoH(out, p, sizeof(int)));
+  return MCV_OK;
+}
+
+// Workgroup size and streams per lane a chain code object was built for (its
+// `mcv_block_size` / `mcv_streams_per_lane` globals; 256 and 1 when absent):
+// one workgroup serves block * spl streams.
+int module_block(hipModule_t mod, int* block, int* spl = nullptr) {
+  int rc = module_int(mod, "mcv_block_size", 256, block);
+  if (rc) return rc;
   if (*block < 64 || *block > 1024 || *block % 64)
     return fail(MCV_ERR_MODULE, "code object built for an unsupported workgroup size %d", *block);
+  if (spl) {
+    rc = module_int(mod, "mcv_streams_per_lane", 1, spl);
+    if (rc) return rc;
+    if (*spl < 1 || *spl > 16) return fail(MCV_ERR_MODULE, "code object built for %d streams per lane", *spl);
+  }
   return MCV_OK;
 }
 
@@ -552,7 +567,7 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   e = hipModuleGetFunction(&h->fn, h->mod, kernel_name);
   if (e != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s: %s", kernel_name, code_object, hipGetErrorString(e));
-  return module_block(h->mod, &h->block);
+  return module_block(h->mod, &h->block, &h->spl);
 }
 
 void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks, int64_t& nthreads, int64_t& npt) {
@@ -643,7 +658,8 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   A.tid_base = 0;
   A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
   const unsigned bs = (unsigned)h->block;
-  const int64_t grid = (nthreads + bs - 1) / bs;
+  const int64_t per_group = (int64_t)h->block * h->spl;
+  const int64_t grid = (nthreads + per_group - 1) / per_group;
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, (unsigned)(A.lds_total * sizeof(double)),
                                  h->stream, kargs, nullptr));
@@ -949,7 +965,7 @@ void mcv_instr_destroy(mcv_instr* h) { release(h); }
 
 // ------------------------------------------------------------------ batched scans
 struct mcv_batch {
-  int ninst = 0, ncomp = 0, block = 256;
+  int ninst = 0, ncomp = 0, block = 256, spl = 1;
   mcv::LaunchArgs args{};
   int64_t pstride = 0, hstride = 0;
   std::vector<int64_t> nbins;
@@ -1042,7 +1058,7 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   std::string bn = std::string(kernel_name) + "_batch";
   if (hipModuleGetFunction(&b->fn_batch, b->mod, bn.c_str()) != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s", bn.c_str(), code_object);
-  return module_block(b->mod, &b->block);
+  return module_block(b->mod, &b->block, &b->spl);
 }
 
 int ensure_rng(mcv_batch* b, int64_t n) {
@@ -1122,7 +1138,8 @@ int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t 
   for (int k = 0; k < n; ++k) {
     int64_t nb, nt, npt;
     partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
-    meta[k + 1] = meta[k] + (nt + b->block - 1) / b->block;
+    const int64_t per_item = (int64_t)b->block * b->spl;
+    meta[k + 1] = meta[k] + (nt + per_item - 1) / per_item;
     meta[n + 1 + k] = (int64_t)ncounts[k];
     meta[2 * n + 1 + k] = nt;
     meta[3 * n + 1 + k] = npt;
@@ -1222,7 +1239,8 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.tid_base = 0;
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
-    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + b->block - 1) / b->block), 1, 1, (unsigned)b->block, 1,
+    const int64_t per_group = (int64_t)b->block * b->spl;
+    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + per_group - 1) / per_group), 1, 1, (unsigned)b->block, 1,
                                    1, (unsigned)(A.lds_total * sizeof(double)), b->stream, kargs,
                                    nullptr));
     HIPCHECK(hipEventRecord(b->ev[1], b->stream));

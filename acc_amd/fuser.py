@@ -98,11 +98,13 @@ def _phase(components, lo, hi, first_transform):
     return "\n  ".join(lines)
 
 
-def generate_source(instrument, compact=True, block=None, waves_per_eu=None):
+def generate_source(instrument, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
     """(source text, structural description) of the fused kernel, for
-    workgroups of `block` threads (default config.kernel_block) and, if
-    given, an occupancy target of `waves_per_eu` waves per SIMD (default
-    config.kernel_waves_per_eu; None leaves it to the compiler)."""
+    workgroups of `block` threads (default config.kernel_block), if given an
+    occupancy target of `waves_per_eu` waves per SIMD (default
+    config.kernel_waves_per_eu; None leaves it to the compiler), and
+    `streams_per_lane` random streams per lane in stream-carrying compacted
+    kernels (default config.kernel_streams_per_lane)."""
     from . import config
     block = int(block or config.kernel_block)
     if block % 64 or not 64 <= block <= 1024:
@@ -114,6 +116,11 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None):
             raise ValueError("waves per SIMD must be in [1, 8], got %r" % wpe)
         defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
     src, structure = _generate(instrument, compact)
+    if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src:
+        spl = int(streams_per_lane or config.kernel_streams_per_lane)
+        if not 1 <= spl <= 16:
+            raise ValueError("streams per lane must be in [1, 16], got %d" % spl)
+        defs += "#define MCV_SPL %d\n" % spl
     return src.replace('#include "mcv_kernel.h"', defs + '#include "mcv_kernel.h"', 1), structure
 
 
@@ -220,7 +227,8 @@ def compile_source(src, verbose=False):
     return out, name
 
 
-def fuse(instrument, verbose=False, compact=True, block=None, waves_per_eu=None):
+def fuse(instrument, verbose=False, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
     """Compile (or fetch from the cache) the fused kernel of an instrument."""
-    src, _ = generate_source(instrument, compact=compact, block=block, waves_per_eu=waves_per_eu)
+    src, _ = generate_source(instrument, compact=compact, block=block, waves_per_eu=waves_per_eu,
+                             streams_per_lane=streams_per_lane)
     return compile_source(src, verbose=verbose)

@@ -246,12 +246,14 @@ def build_all(verbose=False, jobs=None):
     from .fuser_proc import generate_process_source
     for name, make in process_units().items():
         srcs[name + "@process"] = generate_process_source(make())[0]
-    from .autotune import DEFAULT_BLOCKS, DEFAULT_WAVES
+    from .autotune import DEFAULT_BLOCKS, DEFAULT_WAVES, DEFAULT_SPL, _spls
     for name in TUNED:
+        instr = instruments()[name]()
         for block in DEFAULT_BLOCKS:
             for w in DEFAULT_WAVES:
-                srcs["%s@%d/%s" % (name, block, w)] = generate_source(instruments()[name](), block=block,
-                                                                      waves_per_eu=w)[0]
+                for spl in _spls(instr, DEFAULT_SPL):
+                    srcs["%s@%d/%s/%s" % (name, block, w, spl)] = generate_source(
+                        instr, block=block, waves_per_eu=w, streams_per_lane=spl)[0]
     unique = sorted(set(srcs.values()))
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:

@@ -99,7 +99,7 @@ def batch_scan(ninst=64, reps=3):
     ins, n = configs.scan(ninst)
     total = int(n.sum())
     scan, kw = tuned_scan(ins, n, use_cache=False)
-    tuned = {"block": scan.block, "waves_per_eu": scan.waves_per_eu}
+    tuned = {"block": scan.block, "waves_per_eu": scan.waves_per_eu, "streams_per_lane": scan.streams_per_lane}
     try:
         scan.run(n, seed=0, **kw)
         scan.run_sequential(n, seed=0)
@@ -119,7 +119,9 @@ def batch_scan(ninst=64, reps=3):
     singles = [configs.scan_instance(float(e), float(r), "one%d" % k)
                for k, (e, r) in enumerate(zip(*configs.scan_plan(ninst)[:2]))]
     saved = config.kernel_block, config.kernel_waves_per_eu
+    saved_spl = config.kernel_streams_per_lane
     config.kernel_block, config.kernel_waves_per_eu = tuned["block"], tuned["waves_per_eu"]
+    config.kernel_streams_per_lane = tuned["streams_per_lane"]
     try:
         for instr, c in zip(singles, n):  # builds the handles (module loads) outside the timing
             run_instrument(instr, int(min(c, 1000)), seed=0)
@@ -129,6 +131,7 @@ def batch_scan(ninst=64, reps=3):
         api_wall = (time.perf_counter() - t) * 1e3
     finally:
         config.kernel_block, config.kernel_waves_per_eu = saved
+        config.kernel_streams_per_lane = saved_spl
     bk = sum(b) / len(b)
     return {"instances": ninst, "neutrons": total, "ms_batched_graph_kernel": bk, "ms_batched_graph_wall": b_wall,
             "ms_back_to_back_kernels": sum(s) / len(s), "ms_back_to_back_wall": s_wall,
@@ -150,7 +153,7 @@ def adjoint_cost(ncount=int(1e7), reps=3):
     from acc_amd.autotune import tune_instrument
     best = tune_instrument(configs.c5(), ncount, use_cache=False)
     run = DifferentiableRun(configs.c5(), configs.C5_PARAMS, ncount=ncount, seed=0, block=best["block"],
-                            waves_per_eu=best["waves_per_eu"])
+                            waves_per_eu=best["waves_per_eu"], streams_per_lane=best["streams_per_lane"])
     theta = run.theta0()
     g = np.random.default_rng(0).uniform(0.5, 1.5, sum(run.sizes))
     run.forward(theta)
@@ -164,7 +167,8 @@ def adjoint_cost(ncount=int(1e7), reps=3):
     f, b = min(fwd), min(adj)
     return {"workload": "c5 differentiable fit chain, 14 parameters", "ncount": ncount,
             "ms_forward_kernel": f, "ms_adjoint_kernel": b, "adjoint_forward_ratio": b / f,
-            "forward_launch_shape": {"block": best["block"], "waves_per_eu": best["waves_per_eu"]},
+            "forward_launch_shape": {"block": best["block"], "waves_per_eu": best["waves_per_eu"],
+                                     "streams_per_lane": best["streams_per_lane"]},
             "check": "adjoint vs CRN central differences (h=1e-6 rel) of forward kernel and CPU oracle"}
 
 
@@ -201,7 +205,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
-    ap.add_argument("--shape", default=None, help="fixed launch shape BLOCK/WAVES (e.g. 512/None) instead of tuning")
+    ap.add_argument("--shape", default=None, help="fixed launch shape BLOCK/WAVES[/STREAMS_PER_LANE] (e.g. 256/3) instead of tuning")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -221,16 +225,19 @@ def main():
     ncount = int(a.ncount or default_n)
     instr = configs.BUILDERS[a.workload]()
     if a.shape:
-        blk, wv = a.shape.split("/")
-        wv = None if wv in ("None", "", "0") else int(wv)
-        h, shape = get_handle(instr, block=int(blk), waves_per_eu=wv), {"block": int(blk), "waves_per_eu": wv,
-                                                                         "tuned": False}
+        parts = a.shape.split("/") + ["None", "None"]
+        blk = int(parts[0])
+        wv = None if parts[1] in ("None", "", "0") else int(parts[1])
+        spl = None if parts[2] in ("None", "", "0") else int(parts[2])
+        h = get_handle(instr, block=blk, waves_per_eu=wv, streams_per_lane=spl)
+        shape = {"block": blk, "waves_per_eu": wv, "streams_per_lane": spl, "tuned": False}
     elif a.no_tune:
         h, shape = get_handle(instr), {"block": None, "waves_per_eu": None, "tuned": False}
     else:
         h, best = tuned_handle(instr, ncount, use_cache=False)
         shape = {"block": best["block"], "waves_per_eu": best["waves_per_eu"], "tuned": True,
-                 "trials_ms": {"%d/%s" % (t["block"], t["waves_per_eu"]): round(t["ms"], 2)
+                 "streams_per_lane": best["streams_per_lane"],
+                 "trials_ms": {"%d/%s/%s" % (t["block"], t["waves_per_eu"], t["streams_per_lane"]): round(t["ms"], 2)
                                for t in best["trials"]}}
     seed = 0
     for _ in range(a.warmup):

@@ -1,6 +1,6 @@
-"""Launch shape: the workgroup size a chain is compiled for changes nothing in
-the results (each stream's neutrons are computed the same way), and the
-autotuner returns a usable choice."""
+"""Launch shape: the workgroup size and the streams per lane a chain is
+compiled for change nothing in the results (each stream's neutrons are
+computed the same way), and the autotuner returns a usable choice."""
 import numpy as np
 import pytest
 
@@ -11,8 +11,8 @@ from acc_amd.run_script import get_handle
 pytestmark = pytest.mark.gpu
 
 
-def _run(instr, block, N, ntot):
-    h = get_handle(instr, block=block)
+def _run(instr, block, N, ntot, spl=None):
+    h = get_handle(instr, block=block, streams_per_lane=spl)
     h.set_dump(0, N)
     h.clear()
     st = h.run(N, 3, ntot, 512, MCV_FLAG_GATE_DEAD)
@@ -22,14 +22,14 @@ def _run(instr, block, N, ntot):
     return states, alive, hists
 
 
-@pytest.mark.parametrize("name", ["c2", "c2s"])
-@pytest.mark.parametrize("block", [128, 512])
-def test_block_size_invariance(name, block):
-    print(f"\n[launch] {name} block {block}", flush=True)
+@pytest.mark.parametrize("name,block,spl", [("c2", 128, None), ("c2", 512, None), ("c2s", 128, 1), ("c2s", 512, 2),
+                                            ("c2s", 256, 4)])
+def test_block_size_invariance(name, block, spl):
+    print(f"\n[launch] {name} block {block} streams/lane {spl}", flush=True)
     instr = prebuilt.instruments()[name]()
     N, ntot = 300000, 20000
-    s0, a0, h0 = _run(instr, 256, N, ntot)
-    s1, a1, h1 = _run(instr, block, N, ntot)
+    s0, a0, h0 = _run(instr, 256, N, ntot, spl=1 if spl else None)
+    s1, a1, h1 = _run(instr, block, N, ntot, spl=spl)
     np.testing.assert_array_equal(a1, a0)
     np.testing.assert_array_equal(s1, s0)
     for i in h0:
@@ -42,10 +42,11 @@ def test_autotune_scan_small():
     Ei, rot = [30.0, 90.0, 150.0], [0.0, 60.0, 120.0]
     ins = [configs.scan_instance(e, r, "t%d" % k) for k, (e, r) in enumerate(zip(Ei, rot))]
     n = [20000, 150000, 3000]
-    kw = dict(blocks=(128, 256), waves=(None, 3), grids=(0, 256))
+    kw = dict(blocks=(128, 256), waves=(None, 3), spls=(1, 4), grids=(0, 256))
     best = autotune.tune_scan(ins, n, use_cache=False, **kw)
     assert best["block"] in (128, 256) and best["grid"] in (0, 256) and best["waves_per_eu"] in (None, 3)
-    assert len(best["trials"]) == 8 and all(t["ms"] > 0 for t in best["trials"])
+    assert best["streams_per_lane"] in (1, 4)
+    assert len(best["trials"]) == 16 and all(t["ms"] > 0 for t in best["trials"])
     scan, kw = autotune.tuned_scan(ins, n, **kw)
     try:
         st = scan.run(n, seed=1, **kw)
