@@ -28,14 +28,16 @@ class DifferentiableRun:
 
     def __init__(self, instrument, params, ncount, seed=0, monitors=None, ntotalthreads=None,
                  threads_per_block=None, gate_dead_neutrons=True, block=None, waves_per_eu=None,
-                 streams_per_lane=None):
+                 streams_per_lane=None, adjoint_block=None, adjoint_streams_per_lane=None, adjoint_compact=True):
         self.instrument = instrument
         self.ncount, self.seed = int(ncount), int(seed)
         self.ntotalthreads = ntotalthreads or config.ntotalthreads
         self.tpb = threads_per_block or config.threads_per_block
         self.flags = MCV_FLAG_GATE_DEAD if gate_dead_neutrons else 0
         self.h = get_handle(instrument, block=block, waves_per_eu=waves_per_eu, streams_per_lane=streams_per_lane)
-        self.code_object, self.kernel_name, layout = fuse_adjoint(instrument)
+        self.code_object, self.kernel_name, layout = fuse_adjoint(
+            instrument, block=adjoint_block or config.adjoint_block, compact=adjoint_compact,
+            streams_per_lane=adjoint_streams_per_lane or config.adjoint_streams_per_lane)
         comps = instrument.components
         names = [c.name for c in comps]
         # score slots

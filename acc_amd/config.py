@@ -18,6 +18,10 @@ kernel_block = 256
 kernel_waves_per_eu = None
 # random streams per lane in compacted kernels whose later stages draw
 kernel_streams_per_lane = 4
+# launch shape of the compacted adjoint kernels (their queues carry the score
+# payload; on the c5 fit chain 64 x 4 measured fastest of 64/128/256 x 1/2/4)
+adjoint_block = 64
+adjoint_streams_per_lane = 4
 floattype = "float64"
 rng_seed = 1
 gate_dead_neutrons = True

@@ -161,7 +161,134 @@ __device__ __forceinline__ void run_lane_adjoint(const LaunchArgs& A, const Adjo
   }
 }
 
+// Compacted adjoint: the forward kernel's stream-carrying two-phase scheme
+// (run_block_compacted_rng) with the score slots of the phase-A components
+// (Chain::NX doubles) travelling with the queued neutron.  Chain::phase_a(rng,
+// n, A, J, acc, xa, nidx) returns whether the neutron reaches phase B with
+// its phase-A scores in xa; Chain::phase_b(rng, n, A, J, acc, xa) finishes it.
+template <int NX>
+struct QueueAdjLDS : QueueRngLDS {
+  double xt[NX][MCV_QCAP];
+};
+
+template <class Chain>
+__device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const AdjointArgs& J, int64_t tid,
+                                                      QueueAdjLDS<Chain::NX>& q) {
+  const int lt = threadIdx.x;
+  constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
+  double acc[Chain::NS];
+#pragma unroll
+  for (int k = 0; k < Chain::NS; ++k) acc[k] = 0;
+  if (lt == 0) q.count = 0;
+  unsigned busy = 0, done = 0;
+  int64_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < MCV_SPL; ++k) {
+    Rng r;
+    int64_t idx, end;
+    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
+    const int o = lt * MCV_SPL + k;
+    q.st0[o] = r.s0;
+    q.st1[o] = r.s1;
+    q.next[o] = idx;
+    q.returned[o] = 0;
+    mine += end - idx;
+    if (idx >= end) done |= 1u << k;
+  }
+  int par = 0, cnt = 0, rr = 0;
+  __syncthreads();
+  while (true) {
+    const unsigned avail = ~(busy | done) & ALL;
+    bool has = false;
+    Neutron n;
+    n.s1 = 0;
+    n.s2 = 0;
+    double xa[Chain::NX];
+#pragma unroll
+    for (int k = 0; k < Chain::NX; ++k) xa[k] = 0;
+    int sel = -1;
+    if (avail) {
+      const unsigned rot = ((avail >> rr) | (avail << (MCV_SPL - rr))) & ALL;
+      sel = (rr + __builtin_ctz(rot)) % MCV_SPL;
+      const int o = lt * MCV_SPL + sel;
+      Rng w;
+      w.s0 = q.st0[o];
+      w.s1 = q.st1[o];
+      const int64_t my = q.next[o];
+      has = Chain::phase_a(w, n, A, J, acc, xa, my) != 0;
+      q.st0[o] = w.s0;
+      q.st1[o] = w.s1;
+      q.next[o] = my + 1;
+      int64_t s_idx, s_end;
+      Rng unused;
+      stream_range(A, tid + (int64_t)sel * MCV_BLOCK, unused, s_idx, s_end);
+      if (my + 1 >= s_end) done |= 1u << sel;
+      if (has) busy |= 1u << sel;
+      rr = (sel + 1) % MCV_SPL;
+    }
+    const int s = q_append(q, has);
+    if (has) {
+      q_store_state(q, s, n);
+      q.owner[s] = lt * MCV_SPL + sel;
+#pragma unroll
+      for (int k = 0; k < Chain::NX; ++k) q.xt[k][s] = xa[k];
+    }
+    int n_new;
+    bool can_a;
+    round_totals(q, par, has, (~(busy | done) & ALL) != 0, n_new, can_a);
+    cnt += n_new;
+    if (cnt >= MCV_BLOCK || (!can_a && cnt > 0)) {
+      const int take = cnt >= MCV_BLOCK ? MCV_BLOCK : cnt;
+      if (lt < take) {
+        Neutron b;
+        const int slot = cnt - take + lt;
+        q_load_state(q, slot, b);
+        double xb[Chain::NX];
+#pragma unroll
+        for (int k = 0; k < Chain::NX; ++k) xb[k] = q.xt[k][slot];
+        const int o = q.owner[slot];
+        Rng r;
+        r.s0 = q.st0[o];
+        r.s1 = q.st1[o];
+        Chain::phase_b(r, b, A, J, acc, xb);
+        q.st0[o] = r.s0;
+        q.st1[o] = r.s1;
+        q.returned[o] = 1;
+      }
+      cnt -= take;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < MCV_SPL; ++k) {
+        const int o = lt * MCV_SPL + k;
+        if (q.returned[o]) {
+          q.returned[o] = 0;
+          busy &= ~(1u << k);
+        }
+      }
+      if (lt == 0) q.count = cnt;
+      __syncthreads();
+    } else if (!can_a) {
+      break;
+    }
+  }
+  count_processed(A.counter, mine);
+#pragma unroll
+  for (int k = 0; k < Chain::NS; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0) unsafeAtomicAdd(J.grad + k, v);
+  }
+}
+
 }  // namespace mcv
+
+#define MCV_DEFINE_ADJOINT_KERNEL_COMPACTED(NAME, CHAIN)                                                \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A, const mcv::AdjointArgs J) { \
+    __shared__ mcv::QueueAdjLDS<CHAIN::NX> q;                                                          \
+    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x;      \
+    mcv::run_block_adjoint_rng<CHAIN>(A, J, tid, q);                                                   \
+  }
 
 #define MCV_DEFINE_ADJOINT_KERNEL(NAME, CHAIN)                                                          \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A, const mcv::AdjointArgs J) { \

@@ -462,6 +462,7 @@ struct mcv_instr {
   hipFunction_t adj_fn = nullptr;
   std::string adj_key;
   int adj_block = 256;
+  int adj_spl = 1;
   double* d_G = nullptr;
   double* d_grad = nullptr;
   int64_t grad_cap = 0;
@@ -743,7 +744,7 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     }
     if (hipModuleGetFunction(&h->adj_fn, h->adj_mod, kernel_name) != hipSuccess)
       return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
-    int rc = module_block(h->adj_mod, &h->adj_block);
+    int rc = module_block(h->adj_mod, &h->adj_block, &h->adj_spl);
     if (rc) return rc;
     h->adj_key = key;
   }
@@ -804,9 +805,10 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     A.lds_total = 0;
     mcv::AdjointArgs J{h->d_G, h->d_grad, ndense > 0 ? h->d_dense : nullptr};
     const unsigned bs = (unsigned)h->adj_block;
+    const int64_t per_group = (int64_t)h->adj_block * h->adj_spl;
     void* kargs[] = {&A, &J};
-    HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)((nthreads + bs - 1) / bs), 1, 1, bs, 1, 1, 0, h->stream,
-                                   kargs, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)((nthreads + per_group - 1) / per_group), 1, 1, bs, 1, 1, 0,
+                                   h->stream, kargs, nullptr));
     HIPCHECK(hipEventRecord(h->ev[2], h->stream));
   }
   HIPCHECK(hipStreamSynchronize(h->stream));

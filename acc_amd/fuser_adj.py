@@ -22,9 +22,42 @@ def score_layout(instrument):
     return out
 
 
-def generate_adjoint_source(instrument, block=None):
-    """(source text, score layout) of the adjoint kernel."""
+def _monitor_adjoint(c, i, arrays):
+    """Adjoint statements of monitor i: its bin's weight times every score array."""
+    if c.propagate.binning is None:
+        raise NotImplementedError(f"monitor {c.name} has no binning form")
+    upd = []
+    for name, base, size in arrays:
+        for k in range(size):
+            upd.append(f"acc[{base + k}] += c * {name}[{k}];")
+    return ("{\n    int64_t nb;\n    const int64_t b = %s;\n    if (b >= 0) {\n"
+            "      const double c = mcv::adjoint_weight(J.G + A.hoff[%d], nb, b, n.p);\n      %s\n"
+            "      mcv::dense_add(J, dz, c);\n    }\n  }"
+            % (c.propagate.render_binning(c, i), i, "\n      ".join(upd)))
+
+
+def _adjoint_lines(comps, lo, hi, slot, arrays, first_transform, ret):
+    lines = []
+    for i in range(lo, hi):
+        c = comps[i]
+        if i > 0 and (i > lo or first_transform):
+            lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
+        if c.propagate.monitor:
+            lines.append(_monitor_adjoint(c, i, arrays))
+        elif i in slot:
+            lines.append(c.propagate.render_adjoint(c, i, "s%d" % i))
+        else:
+            lines.append(c.propagate.render(c, i))
+        lines.append("if (A.gate && !(n.p > 0)) return%s;" % (" " + ret if ret else ""))
+    return lines
+
+
+def generate_adjoint_source(instrument, block=None, compact=True, streams_per_lane=None):
+    """(source text, score layout) of the adjoint kernel: the stream-carrying
+    compacted form (phase-A scores travel with the queued neutron) when the
+    chain has a compaction point, else one lane per stream."""
     from . import config
+    from .fuser import split_index
     comps = instrument.components
     if not comps:
         raise ValueError("empty instrument")
@@ -46,28 +79,63 @@ def generate_adjoint_source(instrument, block=None):
         base += len(dp)
     ns = base
     pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
-    lines = list(decl) + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz;"]
-    for i, c in enumerate(comps):
-        if i > 0:
-            lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
-        if c.propagate.monitor:
-            if c.propagate.binning is None:
-                raise NotImplementedError(f"monitor {c.name} has no binning form")
-            upd = []
-            for j, (b0, nj) in sorted(slot.items()):
-                for k in range(nj):
-                    upd.append(f"acc[{b0 + k}] += c * s{j}[{k}];")
-            lines.append("{\n    int64_t nb;\n    const int64_t b = %s;\n    if (b >= 0) {\n"
-                         "      const double c = mcv::adjoint_weight(J.G + A.hoff[%d], nb, b, n.p);\n      %s\n"
-                         "      mcv::dense_add(J, dz, c);\n    }\n  }"
-                         % (c.propagate.render_binning(c, i), i, "\n      ".join(upd)))
-        elif i in slot:
-            lines.append(c.propagate.render_adjoint(c, i, "s%d" % i))
-        else:
-            lines.append(c.propagate.render(c, i))
-        lines.append("if (A.gate && !(n.p > 0)) return;")
-    body = "\n  ".join(lines)
     blk = int(block or config.kernel_block)
+    k = split_index(comps) if compact else None
+    if k is not None:
+        spl = int(streams_per_lane or config.kernel_streams_per_lane)
+        # phase-A score arrays are views into the payload xa (xb in phase B)
+        a_slots = [(i, b0, nj) for i, (b0, nj) in sorted(slot.items()) if i <= k]
+        nx, xoff = 0, {}
+        for i, b0, nj in a_slots:
+            xoff[i] = nx
+            nx += nj
+        arrays_a = [("s%d" % i, slot[i][0], slot[i][1]) for i in sorted(slot)]
+        a_views = ["double* s%d = xa + %d;" % (i, xoff[i]) for i, _, _ in a_slots]
+        a_decl = ["double s%d[%d] = {%s};" % (i, nj, ", ".join(["0"] * nj))
+                  for i, (b0, nj) in sorted(slot.items()) if i <= k and i not in xoff]
+        b_views = ["double* s%d = xb + %d;" % (i, xoff[i]) for i, _, _ in a_slots]
+        b_decl = ["double s%d[%d] = {%s};" % (i, nj, ", ".join(["0"] * nj))
+                  for i, (b0, nj) in sorted(slot.items()) if i > k]
+        # scores of phase-B components do not exist yet during phase A
+        arrays_pa = [a for a in arrays_a if int(a[0][1:]) <= k]
+        la = a_views + a_decl + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz; (void)nidx;"]
+        la += _adjoint_lines(comps, 0, k + 1, slot, arrays_pa, False, "0")
+        la.append("return 1;")
+        lb = b_views + b_decl + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz;"]
+        lb += _adjoint_lines(comps, k + 1, len(comps), slot, arrays_a, True, "")
+        src = f"""// adjoint of the fused instrument kernel, compacted after component {k}:
+// {" -> ".join(type(c).__name__ for c in comps)}
+#define MCV_BLOCK {blk}
+#define MCV_SPL {spl}
+#include "mcv_adjoint.h"
+
+namespace {{
+{pre}
+struct AdjChain {{
+  static constexpr int NS = {ns};
+  static constexpr int NX = {max(nx, 1)};
+  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                                 const mcv::AdjointArgs& J, double* acc, double* xa,
+                                                 const int64_t nidx) {{
+  const double* __restrict__ P = A.P;
+  {chr(10).join("  " + l for l in la).strip()}
+  }}
+
+  __device__ static __forceinline__ void phase_b(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                                  const mcv::AdjointArgs& J, double* acc, double* xb) {{
+  const double* __restrict__ P = A.P;
+  {chr(10).join("  " + l for l in lb).strip()}
+  }}
+}};
+}}  // namespace
+
+MCV_DEFINE_ADJOINT_KERNEL_COMPACTED(KERNEL_NAME, AdjChain)
+"""
+        return src, layout
+    arrays = [("s%d" % i, b0, nj) for i, (b0, nj) in sorted(slot.items())]
+    lines = list(decl) + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz;"]
+    lines += _adjoint_lines(comps, 0, len(comps), slot, arrays, False, "")
+    body = "\n  ".join(lines)
     src = f"""// adjoint of the fused instrument kernel: {" -> ".join(type(c).__name__ for c in comps)}
 #define MCV_BLOCK {blk}
 #include "mcv_adjoint.h"
@@ -90,8 +158,9 @@ MCV_DEFINE_ADJOINT_KERNEL(KERNEL_NAME, AdjChain)
     return src, layout
 
 
-def fuse_adjoint(instrument, block=None, verbose=False):
+def fuse_adjoint(instrument, block=None, verbose=False, compact=True, streams_per_lane=None):
     """(code object path, kernel name, score layout) of the adjoint kernel."""
-    src, layout = generate_adjoint_source(instrument, block=block)
+    src, layout = generate_adjoint_source(instrument, block=block, compact=compact,
+                                          streams_per_lane=streams_per_lane)
     path, name = compile_source(src, verbose=verbose)
     return path, name, layout
