@@ -80,7 +80,8 @@ def _spls(instrument, spls):
     """Streams-per-lane candidates: only stream-carrying compacted chains have the knob."""
     from .fuser import generate_source
     src = generate_source(instrument, block=64, waves_per_eu=0)[0]
-    return tuple(spls) if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src else (None,)
+    carried = "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src or "MCV_DEFINE_KERNELS_REFILL_RNG" in src
+    return tuple(spls) if carried else (None,)
 
 
 def tune_instrument(instrument, ncount, seed=0, blocks=DEFAULT_BLOCKS, waves=DEFAULT_WAVES, spls=DEFAULT_SPL,

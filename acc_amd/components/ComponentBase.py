@@ -24,7 +24,7 @@ class DeviceFunction:
     """
 
     def __init__(self, call, stochastic=False, monitor=False, preamble=None, adjoint=None, dparams=None,
-                 binning=None):
+                 binning=None, stepper=None):
         self.call = call
         self.stochastic = stochastic
         self.monitor = monitor
@@ -36,12 +36,19 @@ class DeviceFunction:
         self.adjoint = adjoint
         self.dparams = dparams
         self.binning = binning
+        # a component that can run pass by pass (refilled phase B): dict of
+        # `state` (C++ type) and `init` / `step` / `finish` format strings
+        # ({st} = the state, {n} = the neutron, `tr` = a tracker in scope)
+        self.stepper = stepper
 
     def render(self, comp, i, n="n"):
         return self.call.format(i=i, n=n, **comp._render_context(i))
 
     def render_adjoint(self, comp, i, s, n="n"):
         return self.adjoint.format(i=i, n=n, s=s, dz="dz", **comp._render_context(i))
+
+    def render_step(self, comp, i, part, st="st", n="n"):
+        return self.stepper[part].format(i=i, n=n, st=st, **comp._render_context(i))
 
     def render_binning(self, comp, i, n="n"):
         return self.binning.format(i=i, n=n, **comp._render_context(i))

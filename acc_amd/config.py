@@ -18,6 +18,12 @@ kernel_block = 256
 kernel_waves_per_eu = None
 # random streams per lane in compacted kernels whose later stages draw
 kernel_streams_per_lane = 4
+# compacted kernels whose phase B starts with a pass-by-pass component (the
+# tapered guide) can hand lanes new neutrons while others are still in that
+# component ("refilled" phase B).  Off: on the guide chains it measured slower
+# than the plain compacted kernels (it keeps a neutron, its stream and the
+# component's state live across the refill, which costs occupancy)
+kernel_refill = False
 # launch shape of the compacted adjoint kernels (their queues carry the score
 # payload; on the c5 fit chain 64 x 4 measured fastest of 64/128/256 x 1/2/4)
 adjoint_block = 64

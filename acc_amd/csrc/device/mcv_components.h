@@ -79,8 +79,74 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
   return R;
 }
 
+// ------------------------------------------------------------------ exit-time selection
+// The guides pick the earliest of several exit times t_k = n_k / d_k with a
+// chain of strict "t2 < t1" tests over the walls the neutron moves towards
+// (d_k < 0; guide.py:100-123, guide_tapering.py:206-230).  Here the chain
+// compares quotients by cross-multiplication, without branches: a losing
+// candidate costs three products and two selects instead of an FP64 division
+// (the costliest instruction sequence of the loop, which a wavefront would
+// execute whenever any of its lanes takes the branch).  The winning wall's
+// time is divided afterwards from the same operands as in the reference, so
+// a path is bit for bit the reference's unless two times agree to the last
+// bits (a corner hit).  A zero or non-finite denominator product (vz = 0, an
+// overflow) sets `odd`; the caller then redoes the chain with divisions.
+struct ExitChoice {
+  double n, d;  // the earliest exit time so far is n / d
+  int i;        // 0: the segment (guide) end; 1-4: a wall
+  bool odd;
+};
+
+__device__ __forceinline__ void exit_candidate(ExitChoice& e, double n, double d, int k) {
+  const double s = d * e.d;
+  const double l = n * e.d, r = e.n * d;
+  const bool towards = d < 0;
+  const bool lt = (s > 0 && l < r) || (s < 0 && l > r);
+  e.odd = e.odd || (towards && !(s > 0 || s < 0));
+  const bool take = towards && lt;
+  e.n = take ? n : e.n;
+  e.d = take ? d : e.d;
+  e.i = take ? k : e.i;
+}
+
+// The reference's chain itself, for the rare `odd` case.
+__device__ __forceinline__ void exit_divided(ExitChoice& e, double& t1, double n, double d, int k) {
+  if (d < 0) {
+    const double t2 = n / d;
+    if (t2 < t1) {
+      t1 = t2;
+      e.n = n;
+      e.d = d;
+      e.i = k;
+    }
+  }
+}
+
+__device__ __forceinline__ ExitChoice earliest_exit(double n0, double d0, double n1, double d1, double n2,
+                                                    double d2, double n3, double d3, double n4, double d4) {
+  ExitChoice e{n0, d0, 0, false};
+  exit_candidate(e, n1, d1, 1);
+  exit_candidate(e, n2, d2, 2);
+  exit_candidate(e, n3, d3, 3);
+  exit_candidate(e, n4, d4, 4);
+  if (e.odd) {
+    double t1 = n0 / d0;
+    e.n = n0;
+    e.d = d0;
+    e.i = 0;
+    exit_divided(e, t1, n1, d1, 1);
+    exit_divided(e, t1, n2, d2, 2);
+    exit_divided(e, t1, n3, d3, 3);
+    exit_divided(e, t1, n4, d4, 4);
+  }
+  return e;
+}
+
 // ------------------------------------------------------------------ Guide
-// acc/components/optics/guide.py:59-162.  P = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W
+// acc/components/optics/guide.py:59-162.  P = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W,
+// then the mirror normals' squared lengths and lengths the host derives from
+// them (l*l + ww*ww, its sqrt, l*l + hh*hh, its sqrt: the reference's nlen2
+// and sqrt(nlen2), correctly rounded on either side).
 template <class T>
 __device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P, T tr) {
   const double ww = P[0], hh = P[1], hw1 = P[2], hh1 = P[3], l = P[4];
@@ -98,25 +164,11 @@ __device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P
     const double av = l * vx, bv = ww * vz, ah = l * vy, bh = hh * vz;
     const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
     const double cv1 = -hw1 * l - z * ww, cv2 = x * l, ch1 = -hh1 * l - z * hh, ch2 = y * l;
-    double t1 = (l - z) / vz;
-    int i = 0;
-    if (vdotn_v1 < 0) {
-      const double t2 = (cv1 - cv2) / vdotn_v1;
-      if (t2 < t1) { t1 = t2; i = 1; }
-    }
-    if (vdotn_v2 < 0) {
-      const double t2 = (cv1 + cv2) / vdotn_v2;
-      if (t2 < t1) { t1 = t2; i = 2; }
-    }
-    if (vdotn_h1 < 0) {
-      const double t2 = (ch1 - ch2) / vdotn_h1;
-      if (t2 < t1) { t1 = t2; i = 3; }
-    }
-    if (vdotn_h2 < 0) {
-      const double t2 = (ch1 + ch2) / vdotn_h2;
-      if (t2 < t1) { t1 = t2; i = 4; }
-    }
-    if (i == 0) break;
+    const ExitChoice e = earliest_exit(l - z, vz, cv1 - cv2, vdotn_v1, cv1 + cv2, vdotn_v2, ch1 - ch2, vdotn_h1,
+                                       ch1 + ch2, vdotn_h2);
+    const int i = e.i;
+    if (i == 0) break;  // the guide-end time itself is never used
+    const double t1 = e.n / e.d;
     x += vx * t1;
     y += vy * t1;
     z += vz * t1;
@@ -124,15 +176,13 @@ __device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P
     // reflection off mirror i: the four cases share one form with (axis, sign)
     const bool vert = i <= 2;
     const double wh = vert ? ww : hh;
-    const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
-    const double nlen2 = l * l + wh * wh;
-    const double q = V2K * (-2) * vdn / sqrt(nlen2);
+    const double vdn = e.d;
+    const double nlen2 = vert ? P[10] : P[12];
+    const double q = V2K * (-2) * vdn / (vert ? P[11] : P[13]);
     const double d = 2 * vdn / nlen2;
     const double dl = d * l;
-    if (i == 1) vx = vx - dl;
-    else if (i == 2) vx = vx + dl;
-    else if (i == 3) vy = vy - dl;
-    else vy = vy + dl;
+    vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
+    vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;
     vz = vz - d * wh;
     const double R = calc_reflectivity(q, P[5], P[6], P[7], P[8], P[9]);
     if (R > 0) tr.mirror(vert ? 0 : 1, q, P[5], P[6], P[7], P[8], P[9], 1.0);
@@ -189,140 +239,202 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // ------------------------------------------------------------------ Guide_tapering
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
-// D = 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf).
+// D = 14 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf from
+// the component, then rows the host derives: l_seg^2 + ww^2, its sqrt, l_seg^2 +
+// hh^2, its sqrt -- the reference's nlen2 and sqrt(nlen2) per segment).
 // The reference's two nested loops (segments, then bounces inside a segment)
 // run here as one event loop: each pass computes the next wall or segment-end
 // time of the current segment, then either reflects or moves on to the next
 // segment.  Every neutron still performs exactly the reference's operations in
 // the reference's order, but lanes of a wavefront no longer wait for each
 // other at every segment: a wave runs max(segments + bounces) passes instead
-// of the sum over segments of the per-segment maximum.
+// of the sum over segments of the per-segment maximum.  The loop is exposed as
+// init / step / finish so that a kernel can also hand a lane a new neutron as
+// soon as its previous one leaves the guide (refilled phase B).
+// Reflectivity is the costliest part of a bounce (a division and a tanh), and
+// a wavefront would evaluate it on every pass in which any of its lanes
+// bounces.  A bounce therefore only records its (q, m) as pending.  A lane's
+// pending reflectivity falls due before its next reflection or its exit from
+// the guide; whenever one falls due, every pending lane of the wavefront
+// evaluates its own in the same pass.  Each neutron's weight is still
+// multiplied by its R factors in the reference's order.  A neutron that a
+// pending R kills (R <= 1e-10) may have moved on meanwhile; it ends dead all
+// the same, which is all the reference's early return leaves behind (p = 0,
+// state as it entered).  (Also evaluating when half the wavefront held a
+// pending R measured slower on the 100-segment guide: 26.3 vs 25.8 ms per
+// 1e8 neutrons.)
+
+struct TaperState {
+  double x, y, z, vx, vy, vz, t, prob;
+  double zr, w1s, hhs, wws, hhh, whs, hadj;
+  double pq, pm, pdm;  // pending reflection: q, m and dm/dmxy
+  int seg, nb;
+  int pax;  // pending reflection's mirror axis (0 vertical, 1 horizontal), -1 none
+  bool enter, died;
+};
+
+__device__ __forceinline__ void taper_init(TaperState& s, const Neutron& n, const double* __restrict__ P) {
+  const int nseg = (int)P[9];
+  s.x = n.x; s.y = n.y; s.z = n.z; s.vx = n.vx; s.vy = n.vy; s.vz = n.vz; s.t = n.t; s.prob = n.p;
+  const double dt = -s.z / s.vz;
+  s.x += s.vx * dt;
+  s.y += s.vy * dt;
+  s.z = 0.0;
+  s.t += dt;
+  s.seg = 0;
+  s.nb = 0;
+  s.zr = s.w1s = s.hhs = s.wws = s.hhh = s.whs = s.hadj = 0;
+  s.pq = s.pm = s.pdm = 0;
+  s.pax = -1;
+  s.enter = nseg > 0;
+  s.died = false;
+}
+
+// Applies the pending reflection; returns true when it kills the neutron.
+template <class T>
+__device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __restrict__ P, T& tr) {
+  const bool vert = s.pax == 0;
+  const double Qc = vert ? P[2] : P[3], alpha = vert ? P[4] : P[5];
+  const double R = calc_reflectivity(s.pq, P[1], Qc, alpha, s.pm, P[8]);
+  s.pax = -1;
+  if (R > 1e-10) {
+    tr.mirror(vert ? 0 : 1, s.pq, P[1], Qc, alpha, s.pm, P[8], s.pdm);
+    s.prob *= R;
+    return false;
+  }
+  s.died = true;
+  return true;
+}
+
+// One pass; returns true when the neutron has left the guide or died.
+template <class T>
+__device__ __forceinline__ bool taper_step(TaperState& s, const double* __restrict__ P,
+                                           const double* __restrict__ D, T& tr) {
+  const double l_seg = P[0];
+  const int nseg = (int)P[9];
+  if (s.enter) {  // segment entrance: guide_tapering.py:170-190
+    const double* __restrict__ w1 = D;
+    const double* __restrict__ wwa = D + 4 * nseg;
+    const double* __restrict__ hha = D + 5 * nseg;
+    const double* __restrict__ whalf = D + 6 * nseg;
+    const double* __restrict__ hhalf = D + 7 * nseg;
+    s.zr = (double)s.seg * l_seg;
+    s.w1s = w1[s.seg];
+    s.hhs = hhalf[s.seg];
+    s.wws = wwa[s.seg];
+    s.hhh = hha[s.seg];
+    s.whs = whalf[s.seg];
+    const double dt = (s.zr - s.z) / s.vz;
+    s.x += s.vx * dt;
+    s.y += s.vy * dt;
+    s.z += s.vz * dt;
+    if (s.x <= -0.5 * s.w1s || s.x >= 0.5 * s.w1s || s.y <= -s.hhs || s.y >= s.hhs) {
+      s.died = true;
+      return true;
+    }
+    s.x += 0.5 * s.w1s;
+    // x is now in [0, w1s] (rounding keeps x > -w1s/2 from going negative), so
+    // floor(x / w1s) is 0 unless the quotient rounds up to 1: divide only then
+    const double edge = s.x < 0.75 * s.w1s ? 0.0 : floor(s.x / s.w1s) * s.w1s;
+    if (s.x - edge > s.w1s) {
+      s.died = true;
+      return true;
+    }
+    s.x -= (edge + 0.5 * s.w1s);
+    s.hadj = edge + (0.5 * s.w1s) - 0.5 * s.w1s;
+    s.nb = 0;
+    s.enter = false;
+  } else if (s.seg >= nseg) {
+    return true;
+  }
+  // one pass of the bounce loop (guide_tapering.py:193-284); nb counts the
+  // reference's max_bounces passes per segment
+  bool seg_end = s.nb >= 100000;
+  const double zr = s.zr, wws = s.wws, hhh = s.hhh, z = s.z, x = s.x, y = s.y, vx = s.vx, vy = s.vy, vz = s.vz;
+  const double av = l_seg * vx, bv = wws * vz, ah = l_seg * vy, bh = hhh * vz;
+  const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
+  const double cv1 = -s.whs * l_seg - (z - zr) * wws, cv2 = x * l_seg;
+  const double ch1 = -s.hhs * l_seg - (z - zr) * hhh, ch2 = y * l_seg;
+  ExitChoice e{0.0, 0.0, 0, false};
+  if (!seg_end)
+    e = earliest_exit(zr + l_seg - z, vz, cv1 - cv2, vdotn_v1, cv1 + cv2, vdotn_v2, ch1 - ch2, vdotn_h1, ch1 + ch2,
+                      vdotn_h2);
+  const int i = e.i;
+  // pending reflectivities: due before this lane reflects again or leaves the guide
+  {
+    const bool pend = s.pax >= 0;
+    const bool due = pend && (i != 0 || s.seg + 1 >= nseg);
+    if (__ballot(due) != 0ull && pend && taper_reflect(s, P, tr)) return true;
+  }
+  if (!seg_end) {
+    if (i == 0) {
+      seg_end = true;  // the neutron leaves the segment (the segment-end time is never used)
+    } else {
+      const double t1 = e.n / e.d;
+      s.x = x + vx * t1;
+      s.y = y + vy * t1;
+      s.z = z + vz * t1;
+      s.t += t1;
+      const bool vert = i <= 2;
+      const double wh = vert ? wws : hhh;
+      const double vdn = e.d;
+      const double nlen2 = D[(vert ? 10 : 12) * nseg + s.seg];
+      const double q = V2K * (-2.0) * vdn / D[(vert ? 11 : 13) * nseg + s.seg];
+      const double d = 2.0 * vdn / nlen2;
+      const double dl = d * l_seg;
+      s.vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
+      s.vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;
+      s.vz = vz - d * wh;
+      const double mxy = vert ? P[6] : P[7];
+      if (mxy == 0) {
+        s.x += s.hadj;  // the reference adds hadj here and again at the segment end
+        seg_end = true;
+      } else {
+        const double* __restrict__ w1 = D;
+        const double* __restrict__ h1 = D + 2 * nseg;
+        double m = mxy, dm = 1.0;
+        if (m <= 0.0) {
+          const double ratio = vert ? w1[0] / s.w1s : h1[0] / h1[s.seg];
+          m = vert ? fabs(mxy * w1[0] / s.w1s) : fabs(mxy * h1[0] / h1[s.seg]);
+          dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
+        }
+        s.pq = q;  // R(q, m) is applied by taper_reflect
+        s.pm = m;
+        s.pdm = dm;
+        s.pax = vert ? 0 : 1;
+        ++s.nb;
+      }
+    }
+  }
+  if (seg_end) {
+    s.x += s.hadj;
+    ++s.seg;
+    s.enter = s.seg < nseg;
+    if (!s.enter) return true;
+  }
+  return false;
+}
+
+// A dead neutron keeps the state it entered with (p = 0); a transmitted one
+// takes the guide's final state (guide_tapering.py:286-289).
+__device__ __forceinline__ void taper_finish(const TaperState& s, Neutron& n) {
+  if (s.died) {
+    n.p = 0;
+    return;
+  }
+  n.x = s.x; n.y = s.y; n.z = s.z; n.vx = s.vx; n.vy = s.vy; n.vz = s.vz;
+  n.t = s.t;
+  n.p = s.prob;
+}
+
 template <class T>
 __device__ __forceinline__ void guide_tapering_t(Neutron& n, const double* __restrict__ P,
                                                  const double* __restrict__ D, T tr) {
-  const double l_seg = P[0], R0 = P[1], W = P[8];
-  const int nseg = (int)P[9];
-  const double* __restrict__ w1 = D;
-  const double* __restrict__ h1 = D + 2 * nseg;
-  const double* __restrict__ wwa = D + 4 * nseg;
-  const double* __restrict__ hha = D + 5 * nseg;
-  const double* __restrict__ whalf = D + 6 * nseg;
-  const double* __restrict__ hhalf = D + 7 * nseg;
-  double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz, t = n.t, prob = n.p;
-  double dt = -z / vz;
-  x += vx * dt;
-  y += vy * dt;
-  z = 0.0;
-  t += dt;
-  int seg = 0, nb = 0;
-  double zr = 0, w1s = 0, hhs = 0, wws = 0, hhh = 0, whs = 0, hadj = 0;
-  bool enter = nseg > 0;
-  while (true) {
-    if (enter) {  // segment entrance: guide_tapering.py:170-190
-      zr = (double)seg * l_seg;
-      w1s = w1[seg];
-      hhs = hhalf[seg];
-      wws = wwa[seg];
-      hhh = hha[seg];
-      whs = whalf[seg];
-      dt = (zr - z) / vz;
-      x += vx * dt;
-      y += vy * dt;
-      z += vz * dt;
-      if (x <= -0.5 * w1s || x >= 0.5 * w1s || y <= -hhs || y >= hhs) {
-        n.p = 0;
-        return;
-      }
-      x += 0.5 * w1s;
-      const double edge = floor(x / w1s) * w1s;
-      if (x - edge > w1s) {
-        n.p = 0;
-        return;
-      }
-      x -= (edge + 0.5 * w1s);
-      hadj = edge + (0.5 * w1s) - 0.5 * w1s;
-      nb = 0;
-      enter = false;
-    } else if (seg >= nseg) {
-      break;
-    }
-    // one pass of the bounce loop (guide_tapering.py:193-284); nb counts the
-    // reference's max_bounces passes per segment
-    bool seg_end = nb >= 100000;
-    if (!seg_end) {
-      const double av = l_seg * vx, bv = wws * vz, ah = l_seg * vy, bh = hhh * vz;
-      const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
-      const double cv1 = -whs * l_seg - (z - zr) * wws, cv2 = x * l_seg;
-      const double ch1 = -hhs * l_seg - (z - zr) * hhh, ch2 = y * l_seg;
-      double t1 = (zr + l_seg - z) / vz;
-      int i = 0;
-      if (vdotn_v1 < 0) {
-        const double t2 = (cv1 - cv2) / vdotn_v1;
-        if (t2 < t1) { t1 = t2; i = 1; }
-      }
-      if (vdotn_v2 < 0) {
-        const double t2 = (cv1 + cv2) / vdotn_v2;
-        if (t2 < t1) { t1 = t2; i = 2; }
-      }
-      if (vdotn_h1 < 0) {
-        const double t2 = (ch1 - ch2) / vdotn_h1;
-        if (t2 < t1) { t1 = t2; i = 3; }
-      }
-      if (vdotn_h2 < 0) {
-        const double t2 = (ch1 + ch2) / vdotn_h2;
-        if (t2 < t1) { t1 = t2; i = 4; }
-      }
-      if (i == 0) {
-        seg_end = true;  // the neutron leaves the segment
-      } else {
-        x += vx * t1;
-        y += vy * t1;
-        z += vz * t1;
-        t += t1;
-        const bool vert = i <= 2;
-        const double wh = vert ? wws : hhh;
-        const double vdn = i == 1 ? vdotn_v1 : i == 2 ? vdotn_v2 : i == 3 ? vdotn_h1 : vdotn_h2;
-        const double nlen2 = l_seg * l_seg + wh * wh;
-        const double q = V2K * (-2.0) * vdn / sqrt(nlen2);
-        const double d = 2.0 * vdn / nlen2;
-        const double dl = d * l_seg;
-        if (i == 1) vx = vx - dl;
-        else if (i == 2) vx = vx + dl;
-        else if (i == 3) vy = vy - dl;
-        else vy = vy + dl;
-        vz = vz - d * wh;
-        const double mxy = vert ? P[6] : P[7];
-        if (mxy == 0) {
-          x += hadj;  // the reference adds hadj here and again at the segment end
-          seg_end = true;
-        } else {
-          double m = mxy, dm = 1.0;
-          if (m <= 0.0) {
-            const double ratio = vert ? w1[0] / w1s : h1[0] / h1[seg];
-            m = vert ? fabs(mxy * w1[0] / w1s) : fabs(mxy * h1[0] / h1[seg]);
-            dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
-          }
-          const double R = calc_reflectivity(q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W);
-          if (R > 1e-10) {
-            tr.mirror(vert ? 0 : 1, q, R0, vert ? P[2] : P[3], vert ? P[4] : P[5], m, W, dm);
-            prob *= R;
-          } else {
-            n.p = 0;
-            return;
-          }
-          ++nb;
-        }
-      }
-    }
-    if (seg_end) {
-      x += hadj;
-      ++seg;
-      enter = seg < nseg;
-    }
+  TaperState s;
+  taper_init(s, n, P);
+  while (!taper_step(s, P, D, tr)) {
   }
-  n.x = x; n.y = y; n.z = z; n.vx = vx; n.vy = vy; n.vz = vz;
-  n.t = t;
-  n.p = prob;
+  taper_finish(s, n);
 }
 
 __device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restrict__ P,

@@ -217,6 +217,31 @@ struct Packed {
   int64_t nbins = 0;
 };
 
+// Mirror-normal lengths the guides use at every reflection (guide.py:131-155,
+// guide_tapering.py:240-268: nlen2 = l*l + w*w and sqrt(nlen2)), derived once
+// here from the reference-order parameters.  Host IEEE sqrt is correctly
+// rounded, so the values are the reference's.
+void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, ..., l at [4])
+  const double l = P[4];
+  const double l2 = l * l, w2 = P[0] * P[0], h2 = P[1] * P[1];
+  out[0] = l2 + w2;
+  out[1] = std::sqrt(out[0]);
+  out[2] = l2 + h2;
+  out[3] = std::sqrt(out[2]);
+}
+
+void taper_norms(double l_seg, int64_t nseg, double* D) {  // D: 14 x nseg, rows 0-9 given
+  const double l2 = l_seg * l_seg;
+  for (int64_t k = 0; k < nseg; ++k) {
+    const double ww = D[4 * nseg + k], hh = D[5 * nseg + k];
+    const double w2 = ww * ww, h2 = hh * hh;
+    D[10 * nseg + k] = l2 + w2;
+    D[11 * nseg + k] = std::sqrt(D[10 * nseg + k]);
+    D[12 * nseg + k] = l2 + h2;
+    D[13 * nseg + k] = std::sqrt(D[12 * nseg + k]);
+  }
+}
+
 void cross3(const double* a, const double* b, double* o) {
   o[0] = a[1] * b[2] - a[2] * b[1];
   o[1] = a[2] * b[0] - a[0] * b[2];
@@ -273,6 +298,8 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
     case MCV_GUIDE:
       if ((rc = expect(d, i, 10, 0))) return rc;
       copyp(10);
+      out.params.resize(14);
+      guide_norms(out.params.data(), out.params.data() + 10);
       return MCV_OK;
     case MCV_SLIT:
     case MCV_BEAMSTOP:
@@ -299,6 +326,8 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
       copyp(9);
       out.params.push_back((double)nseg);
       out.data.assign(d.data, d.data + d.n_data);
+      out.data.resize(14 * nseg);
+      taper_norms(d.params[0], nseg, out.data.data());
       return MCV_OK;
     }
     case MCV_SAMPLE_HSS:
@@ -616,6 +645,18 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
   const int64_t off = h->args.off[comp];
   std::memcpy(h->blob.data() + off, params, sizeof(double) * n_params);
   HIPCHECK(hipMemcpy(h->d_blob + off, params, sizeof(double) * n_params, hipMemcpyHostToDevice));
+  // values derived from the parameters follow them
+  if (h->types[comp] == MCV_GUIDE) {
+    double* P = h->blob.data() + off;
+    guide_norms(P, P + 10);
+    HIPCHECK(hipMemcpy(h->d_blob + off + 10, P + 10, sizeof(double) * 4, hipMemcpyHostToDevice));
+  } else if (h->types[comp] == MCV_GUIDE_TAPERING) {
+    const int64_t nseg = (int64_t)h->blob[off + 9];
+    double* D = h->blob.data() + h->args.doff[comp];
+    taper_norms(params[0], nseg, D);
+    HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp] + 10 * nseg, D + 10 * nseg, sizeof(double) * 4 * nseg,
+                       hipMemcpyHostToDevice));
+  }
   return MCV_OK;
 }
 

@@ -233,6 +233,8 @@ def instruments():
 
 # chains also built for every launch shape the autotuner tries
 TUNED = ("c2", "c2s", "c5")
+# chains also built with the opt-in refilled phase B (its invariance test)
+REFILLED = ("c2", "c2s")
 
 
 def build_all(verbose=False, jobs=None):
@@ -254,6 +256,8 @@ def build_all(verbose=False, jobs=None):
                 for spl in _spls(instr, DEFAULT_SPL):
                     srcs["%s@%d/%s/%s" % (name, block, w, spl)] = generate_source(
                         instr, block=block, waves_per_eu=w, streams_per_lane=spl)[0]
+    for name in REFILLED:
+        srcs[name + "@refill"] = generate_source(instruments()[name](), refill=True)[0]
     unique = sorted(set(srcs.values()))
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:
