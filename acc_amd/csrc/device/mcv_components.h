@@ -239,9 +239,10 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // ------------------------------------------------------------------ Guide_tapering
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
-// D = 14 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf from
-// the component, then rows the host derives: l_seg^2 + ww^2, its sqrt, l_seg^2 +
-// hh^2, its sqrt -- the reference's nlen2 and sqrt(nlen2) per segment).
+// D = 16 x nseg table: w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf from
+// the component, then rows the host derives per segment: l_seg^2 + ww^2 and its
+// sqrt, l_seg^2 + hh^2 and its sqrt (the reference's nlen2 and sqrt(nlen2)),
+// whalf * l_seg and hhalf * l_seg (the reference's products in cv1 and ch1).
 // The reference's two nested loops (segments, then bounces inside a segment)
 // run here as one event loop: each pass computes the next wall or segment-end
 // time of the current segment, then either reflects or moves on to the next
@@ -251,9 +252,10 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // of the sum over segments of the per-segment maximum.  The loop is exposed as
 // init / step / finish so that a kernel can also hand a lane a new neutron as
 // soon as its previous one leaves the guide (refilled phase B).
-// Reflectivity is the costliest part of a bounce (a division and a tanh), and
-// a wavefront would evaluate it on every pass in which any of its lanes
-// bounces.  A bounce therefore only records its (q, m) as pending.  A lane's
+// Reflectivity is the costliest part of a bounce (a division for q, another in
+// R and a tanh), and a wavefront would evaluate it on every pass in which any
+// of its lanes bounces.  A bounce therefore only records its wall's v.n, its
+// segment and its axis as pending.  A lane's
 // pending reflectivity falls due before its next reflection or its exit from
 // the guide; whenever one falls due, every pending lane of the wavefront
 // evaluates its own in the same pass.  Each neutron's weight is still
@@ -266,10 +268,10 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 
 struct TaperState {
   double x, y, z, vx, vy, vz, t, prob;
-  double zr, w1s, hhs, wws, hhh, whs, hadj;
-  double pq, pm, pdm;  // pending reflection: q, m and dm/dmxy
+  double zr, wws, hhh, lwh, lhh, hadj;  // current segment (entrance z, slopes, l_seg x half sizes, offset)
+  double pvdn;                          // pending reflection: v.n of its wall
   int seg, nb;
-  int pax;  // pending reflection's mirror axis (0 vertical, 1 horizontal), -1 none
+  int pseg, pax;  // pending reflection's segment and axis (0 vertical, 1 horizontal; -1 none)
   bool enter, died;
 };
 
@@ -283,22 +285,36 @@ __device__ __forceinline__ void taper_init(TaperState& s, const Neutron& n, cons
   s.t += dt;
   s.seg = 0;
   s.nb = 0;
-  s.zr = s.w1s = s.hhs = s.wws = s.hhh = s.whs = s.hadj = 0;
-  s.pq = s.pm = s.pdm = 0;
+  s.zr = s.wws = s.hhh = s.lwh = s.lhh = s.hadj = 0;
+  s.pvdn = 0;
+  s.pseg = 0;
   s.pax = -1;
   s.enter = nseg > 0;
   s.died = false;
 }
 
-// Applies the pending reflection; returns true when it kills the neutron.
+// Applies the pending reflection (guide_tapering.py:240-284); returns true
+// when it kills the neutron.
 template <class T>
-__device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __restrict__ P, T& tr) {
+__device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __restrict__ P,
+                                              const double* __restrict__ D, T& tr) {
+  const int nseg = (int)P[9];
   const bool vert = s.pax == 0;
+  const double q = V2K * (-2.0) * s.pvdn / D[(vert ? 11 : 13) * nseg + s.pseg];
+  const double mxy = vert ? P[6] : P[7];
+  double m = mxy, dm = 1.0;
+  if (m <= 0.0) {
+    const double* __restrict__ w1 = D;
+    const double* __restrict__ h1 = D + 2 * nseg;
+    const double ratio = vert ? w1[0] / w1[s.pseg] : h1[0] / h1[s.pseg];
+    m = vert ? fabs(mxy * w1[0] / w1[s.pseg]) : fabs(mxy * h1[0] / h1[s.pseg]);
+    dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
+  }
   const double Qc = vert ? P[2] : P[3], alpha = vert ? P[4] : P[5];
-  const double R = calc_reflectivity(s.pq, P[1], Qc, alpha, s.pm, P[8]);
+  const double R = calc_reflectivity(q, P[1], Qc, alpha, m, P[8]);
   s.pax = -1;
   if (R > 1e-10) {
-    tr.mirror(vert ? 0 : 1, s.pq, P[1], Qc, alpha, s.pm, P[8], s.pdm);
+    tr.mirror(vert ? 0 : 1, q, P[1], Qc, alpha, m, P[8], dm);
     s.prob *= R;
     return false;
   }
@@ -313,35 +329,30 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   const double l_seg = P[0];
   const int nseg = (int)P[9];
   if (s.enter) {  // segment entrance: guide_tapering.py:170-190
-    const double* __restrict__ w1 = D;
-    const double* __restrict__ wwa = D + 4 * nseg;
-    const double* __restrict__ hha = D + 5 * nseg;
-    const double* __restrict__ whalf = D + 6 * nseg;
-    const double* __restrict__ hhalf = D + 7 * nseg;
+    const double w1s = D[s.seg], hhs = D[7 * nseg + s.seg];
     s.zr = (double)s.seg * l_seg;
-    s.w1s = w1[s.seg];
-    s.hhs = hhalf[s.seg];
-    s.wws = wwa[s.seg];
-    s.hhh = hha[s.seg];
-    s.whs = whalf[s.seg];
+    s.wws = D[4 * nseg + s.seg];
+    s.hhh = D[5 * nseg + s.seg];
+    s.lwh = D[14 * nseg + s.seg];
+    s.lhh = D[15 * nseg + s.seg];
     const double dt = (s.zr - s.z) / s.vz;
     s.x += s.vx * dt;
     s.y += s.vy * dt;
     s.z += s.vz * dt;
-    if (s.x <= -0.5 * s.w1s || s.x >= 0.5 * s.w1s || s.y <= -s.hhs || s.y >= s.hhs) {
+    if (s.x <= -0.5 * w1s || s.x >= 0.5 * w1s || s.y <= -hhs || s.y >= hhs) {
       s.died = true;
       return true;
     }
-    s.x += 0.5 * s.w1s;
+    s.x += 0.5 * w1s;
     // x is now in [0, w1s] (rounding keeps x > -w1s/2 from going negative), so
     // floor(x / w1s) is 0 unless the quotient rounds up to 1: divide only then
-    const double edge = s.x < 0.75 * s.w1s ? 0.0 : floor(s.x / s.w1s) * s.w1s;
-    if (s.x - edge > s.w1s) {
+    const double edge = s.x < 0.75 * w1s ? 0.0 : floor(s.x / w1s) * w1s;
+    if (s.x - edge > w1s) {
       s.died = true;
       return true;
     }
-    s.x -= (edge + 0.5 * s.w1s);
-    s.hadj = edge + (0.5 * s.w1s) - 0.5 * s.w1s;
+    s.x -= (edge + 0.5 * w1s);
+    s.hadj = edge + (0.5 * w1s) - 0.5 * w1s;
     s.nb = 0;
     s.enter = false;
   } else if (s.seg >= nseg) {
@@ -353,8 +364,8 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   const double zr = s.zr, wws = s.wws, hhh = s.hhh, z = s.z, x = s.x, y = s.y, vx = s.vx, vy = s.vy, vz = s.vz;
   const double av = l_seg * vx, bv = wws * vz, ah = l_seg * vy, bh = hhh * vz;
   const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
-  const double cv1 = -s.whs * l_seg - (z - zr) * wws, cv2 = x * l_seg;
-  const double ch1 = -s.hhs * l_seg - (z - zr) * hhh, ch2 = y * l_seg;
+  const double cv1 = -s.lwh - (z - zr) * wws, cv2 = x * l_seg;
+  const double ch1 = -s.lhh - (z - zr) * hhh, ch2 = y * l_seg;
   ExitChoice e{0.0, 0.0, 0, false};
   if (!seg_end)
     e = earliest_exit(zr + l_seg - z, vz, cv1 - cv2, vdotn_v1, cv1 + cv2, vdotn_v2, ch1 - ch2, vdotn_h1, ch1 + ch2,
@@ -364,7 +375,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   {
     const bool pend = s.pax >= 0;
     const bool due = pend && (i != 0 || s.seg + 1 >= nseg);
-    if (__ballot(due) != 0ull && pend && taper_reflect(s, P, tr)) return true;
+    if (__ballot(due) != 0ull && pend && taper_reflect(s, P, D, tr)) return true;
   }
   if (!seg_end) {
     if (i == 0) {
@@ -378,9 +389,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
       const bool vert = i <= 2;
       const double wh = vert ? wws : hhh;
       const double vdn = e.d;
-      const double nlen2 = D[(vert ? 10 : 12) * nseg + s.seg];
-      const double q = V2K * (-2.0) * vdn / D[(vert ? 11 : 13) * nseg + s.seg];
-      const double d = 2.0 * vdn / nlen2;
+      const double d = 2.0 * vdn / D[(vert ? 10 : 12) * nseg + s.seg];
       const double dl = d * l_seg;
       s.vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
       s.vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;
@@ -390,17 +399,8 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
         s.x += s.hadj;  // the reference adds hadj here and again at the segment end
         seg_end = true;
       } else {
-        const double* __restrict__ w1 = D;
-        const double* __restrict__ h1 = D + 2 * nseg;
-        double m = mxy, dm = 1.0;
-        if (m <= 0.0) {
-          const double ratio = vert ? w1[0] / s.w1s : h1[0] / h1[s.seg];
-          m = vert ? fabs(mxy * w1[0] / s.w1s) : fabs(mxy * h1[0] / h1[s.seg]);
-          dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
-        }
-        s.pq = q;  // R(q, m) is applied by taper_reflect
-        s.pm = m;
-        s.pdm = dm;
+        s.pvdn = vdn;  // q, m and R are evaluated by taper_reflect
+        s.pseg = s.seg;
         s.pax = vert ? 0 : 1;
         ++s.nb;
       }

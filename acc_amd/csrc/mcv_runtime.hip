@@ -230,7 +230,10 @@ void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, .
   out[3] = std::sqrt(out[2]);
 }
 
-void taper_norms(double l_seg, int64_t nseg, double* D) {  // D: 14 x nseg, rows 0-9 given
+// Tapered guide, per segment (D: 16 x nseg, rows 0-9 given): the mirror norms
+// as above, and whalf * l_seg, hhalf * l_seg (the products of cv1 and ch1,
+// guide_tapering.py:202-204).
+void taper_norms(double l_seg, int64_t nseg, double* D) {
   const double l2 = l_seg * l_seg;
   for (int64_t k = 0; k < nseg; ++k) {
     const double ww = D[4 * nseg + k], hh = D[5 * nseg + k];
@@ -239,6 +242,8 @@ void taper_norms(double l_seg, int64_t nseg, double* D) {  // D: 14 x nseg, rows
     D[11 * nseg + k] = std::sqrt(D[10 * nseg + k]);
     D[12 * nseg + k] = l2 + h2;
     D[13 * nseg + k] = std::sqrt(D[12 * nseg + k]);
+    D[14 * nseg + k] = D[6 * nseg + k] * l_seg;
+    D[15 * nseg + k] = D[7 * nseg + k] * l_seg;
   }
 }
 
@@ -326,7 +331,7 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
       copyp(9);
       out.params.push_back((double)nseg);
       out.data.assign(d.data, d.data + d.n_data);
-      out.data.resize(14 * nseg);
+      out.data.resize(16 * nseg);
       taper_norms(d.params[0], nseg, out.data.data());
       return MCV_OK;
     }
@@ -654,7 +659,7 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
     const int64_t nseg = (int64_t)h->blob[off + 9];
     double* D = h->blob.data() + h->args.doff[comp];
     taper_norms(params[0], nseg, D);
-    HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp] + 10 * nseg, D + 10 * nseg, sizeof(double) * 4 * nseg,
+    HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp] + 10 * nseg, D + 10 * nseg, sizeof(double) * 6 * nseg,
                        hipMemcpyHostToDevice));
   }
   return MCV_OK;
