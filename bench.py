@@ -185,12 +185,20 @@ def copy_kernel(nbytes=1 << 31, reps=3):
     return 0
 
 
-def traffic_from_profiles(workload):
+def traffic_from_profiles(workload, shape):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE for
+    this workload's kernel at this launch shape (the shapes' spills differ), or
+    None when that shape was not measured."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        shapes = json.load(f).get("shapes", {})
+    key = "%s/%s" % (shape["block"], shape["waves_per_eu"])
+    if shape.get("streams_per_lane"):
+        key += "/%s" % shape["streams_per_lane"]
+    entry = shapes.get(key)
+    return entry["hbm_bytes_per_launch"] if entry else None
 
 
 def main():
@@ -257,7 +265,7 @@ def main():
     k_avg = sum(kernel_ms) / len(kernel_ms)
     achieved = fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles(a.workload),
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_from_profiles(a.workload, shape),
                 "kernel": "mcv_fused (%s)" % a.workload, "kernel_ms_avg": k_avg,
                 "falg_ops_per_neutron": fl["F_alg"], "falg_detail": fl,
                 "rng_seed_kernel_ms_avg": sum(rng_ms) / len(rng_ms)}
