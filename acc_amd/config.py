@@ -24,6 +24,12 @@ kernel_streams_per_lane = 4
 # than the plain compacted kernels (it keeps a neutron, its stream and the
 # component's state live across the refill, which costs occupancy)
 kernel_refill = False
+# launch shape of chains with a multiple-scattering sample: their scatterer
+# needs the whole register file (occupancy 1 unconstrained); a target of 2
+# waves per SIMD trades spills for latency hiding (c3, 1e8 neutrons: 78 ms at
+# 128 / 2 / 4 streams per lane vs 98 ms at 256 / compiler's choice / 4)
+ms_kernel_block = 128
+ms_kernel_waves_per_eu = 2
 # launch shape of the compacted adjoint kernels (their queues carry the score
 # payload; on the c5 fit chain 64 x 4 measured fastest of 64/128/256 x 1/2/4)
 adjoint_block = 64

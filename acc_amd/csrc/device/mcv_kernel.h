@@ -201,6 +201,11 @@ __device__ __forceinline__ void run_lane_process_ms(const LaunchArgs& A, int64_t
 // prefix count, one LDS atomic per wave), and phase B runs over the queue with
 // all lanes of whole wavefronts busy instead of one in four.
 //
+// Chain::phase_a returns r > 0 to queue the neutron; r <= 0 when it is done in
+// phase A, with -r its dump flag (0 for a dropped neutron; a multiple-
+// scattering chain whose neutron misses the sample finishes its one output in
+// phase A and returns -1).  Chain::phase_b returns the dump flag.
+//
 // Control decisions need the workgroup-wide queue length and whether any lane
 // still has work; each wave publishes its two counts in a per-round slot
 // (double-buffered by round parity, waves are never more than one round
@@ -311,8 +316,9 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
     n.s2 = 0;
     const int64_t my = idx;
     if (idx < end) {
-      has = Chain::phase_a(rng, n, A, my) != 0;
-      if (!has) dump_neutron(A, my, n, 0);
+      const int r = Chain::phase_a(rng, n, A, my);
+      has = r > 0;
+      if (!has) dump_neutron(A, my, n, -r);
       ++idx;
     }
     const int s = q_append(q, has);
@@ -389,8 +395,9 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
       w.s0 = q.st0[o];
       w.s1 = q.st1[o];
       my = q.next[o];
-      has = Chain::phase_a(w, n, A, my) != 0;
-      if (!has) dump_neutron(A, my, n, 0);
+      const int r = Chain::phase_a(w, n, A, my);
+      has = r > 0;
+      if (!has) dump_neutron(A, my, n, -r);
       q.st0[o] = w.s0;
       q.st1[o] = w.s1;
       q.next[o] = my + 1;
@@ -555,8 +562,9 @@ __device__ __forceinline__ void run_block_compacted_refill(const LaunchArgs& A, 
     n.s2 = 0;
     const int64_t my = idx;
     if (idx < end) {
-      has = Chain::phase_a(rng, n, A, my) != 0;
-      if (!has) dump_neutron(A, my, n, 0);
+      const int r = Chain::phase_a(rng, n, A, my);
+      has = r > 0;
+      if (!has) dump_neutron(A, my, n, -r);
       ++idx;
     }
     const int s = q_append(q, has);
@@ -622,8 +630,9 @@ __device__ __forceinline__ void run_block_compacted_rng_refill(const LaunchArgs&
       w.s0 = q.st0[o];
       w.s1 = q.st1[o];
       const int64_t my = q.next[o];
-      has = Chain::phase_a(w, n, A, my) != 0;
-      if (!has) dump_neutron(A, my, n, 0);
+      const int r = Chain::phase_a(w, n, A, my);
+      has = r > 0;
+      if (!has) dump_neutron(A, my, n, -r);
       q.st0[o] = w.s0;
       q.st1[o] = w.s1;
       q.next[o] = my + 1;

@@ -104,8 +104,14 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
     occupancy target of `waves_per_eu` waves per SIMD (default
     config.kernel_waves_per_eu; None leaves it to the compiler), and
     `streams_per_lane` random streams per lane in stream-carrying compacted
-    kernels (default config.kernel_streams_per_lane)."""
+    kernels (default config.kernel_streams_per_lane).  Chains with a
+    multiple-scattering sample default to config.ms_kernel_block and
+    config.ms_kernel_waves_per_eu instead."""
     from . import config
+    if any(c.is_multiplescattering for c in instrument.components):
+        block = block or config.ms_kernel_block
+        if waves_per_eu is None:
+            waves_per_eu = config.ms_kernel_waves_per_eu
     block = int(block or config.kernel_block)
     if block % 64 or not 64 <= block <= 1024:
         raise ValueError("workgroup size must be a multiple of 64 in [64, 1024], got %d" % block)
@@ -197,7 +203,7 @@ def _generate(instrument, compact, refill=True):
         assert isinstance(c, ComponentBase), f"{c} is not an acc_amd component"
     if any(c.is_multiplescattering for c in comps):
         from .fuser_ms import generate_ms_source
-        return generate_ms_source(instrument)
+        return generate_ms_source(instrument, compact=compact)
     structure = " -> ".join(type(c).__name__ for c in comps)
     pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
     k = split_index(comps) if compact else None

@@ -6,10 +6,96 @@ change, on each output neutron (acc/run_script.py:204-230, template :245-260).
 The same structure is emitted here; the dump records output 0's final state and
 the number of outputs.  As in the reference, one multiple-scattering component
 per chain is supported (run_script.py:211).
+
+Compacted form (the default): in a wavefront most neutrons miss the sample
+(a hollow cylinder 3 cm across in a 10 cm beam), and the scatterer's bounded
+breadth-first loops would run with the few lanes that hit it.  The scatterer's
+first step decides a miss without drawing (its neutron passes as the single
+output, homogeneous_multiple_scatterer.py:214-217 via is_exiting), so phase A
+runs the components before the sample, that test, and for a miss the
+downstream components on the one output; a neutron that will interact is
+queued with its random stream (stream-carrying compaction: the stream pauses
+until the neutron is through), and phase B runs the scatterer and the
+downstream components on each output with whole wavefronts of interacting
+neutrons.
 """
 
 
-def generate_ms_source(instrument):
+def _tail(comps, m, n="o"):
+    tail = []
+    for i in range(m + 1, len(comps)):
+        c = comps[i]
+        tail.append(f"mcv::apply_transformation({n}, P + A.xoff[{i - 1}]);")
+        tail.append(c.propagate.render(c, i, n=n))
+        tail.append(f"if (A.gate && !({n}.p > 0)) break;")
+    return "\n      ".join(tail)
+
+
+def _generate_compacted(comps, m, pre, structure):
+    a = []
+    for i in range(m):
+        c = comps[i]
+        if i > 0:
+            a.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
+        a.append(c.propagate.render(c, i))
+        a.append("if (A.gate && !(n.p > 0)) return 0;")
+    if m > 0:
+        a.append(f"mcv::apply_transformation(n, P + A.xoff[{m - 1}]);")
+    shape = comps[m]._render_context(m)["shape"]
+    a.append(f"""if (!mcv::is_exiting<{shape}>(P + A.doff[{m}], n)) return 1;  // interacts: phase B
+  {{  // a miss: the neutron is the scatterer's one output
+    mcv::Neutron o = n;
+    do {{
+      if (A.gate && !(o.p > 0)) break;
+      {_tail(comps, m)}
+    }} while (0);
+    n = o;
+  }}
+  return -1;""")
+    M = comps[m].max_scattered_neutrons
+    b = [f"mcv::Neutron out[{M}];",
+         "const int num = " + comps[m].propagate.render(comps[m], m) + ";",
+         f"""for (int k = 0; k < num; ++k) {{
+    mcv::Neutron o = out[k];
+    do {{
+      if (A.gate && !(o.p > 0)) break;
+      {_tail(comps, m)}
+    }} while (0);
+    if (k == 0) n = o;
+  }}""",
+         "return num;"]
+    a_text = "\n  ".join(a)
+    b_text = "\n  ".join(b)
+    src = f"""// fused instrument kernel (multiple scattering), compacted at the sample: {structure}
+#include "mcv_kernel.h"
+#include "mcv_hms.h"
+
+namespace {{
+{pre}
+struct Chain {{
+  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                                 const int64_t nidx) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)rng; (void)H; (void)nidx;
+  {a_text}
+  }}
+
+  __device__ static __forceinline__ int phase_b(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  const double* __restrict__ P = A.P;
+  double* __restrict__ H = A.H;
+  (void)H;
+  {b_text}
+  }}
+}};
+}}  // namespace
+
+MCV_DEFINE_KERNELS_COMPACTED_RNG(KERNEL_NAME, Chain)
+"""
+    return src, structure + " [compacted at the sample, streams carried]"
+
+
+def generate_ms_source(instrument, compact=True):
     comps = instrument.components
     ms = [i for i, c in enumerate(comps) if c.is_multiplescattering]
     if len(ms) != 1:
@@ -17,6 +103,8 @@ def generate_ms_source(instrument):
     m = ms[0]
     structure = " -> ".join(type(c).__name__ for c in comps)
     pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
+    if compact:
+        return _generate_compacted(comps, m, pre, structure)
     lines = []
     for i in range(m):
         c = comps[i]

@@ -235,6 +235,8 @@ def instruments():
 TUNED = ("c2", "c2s", "c5")
 # chains also built with the opt-in refilled phase B (its invariance test)
 REFILLED = ("c2", "c2s")
+# multiple-scattering chains also built uncompacted (their compaction invariance test)
+UNCOMPACTED = ("c3-small", "hms-ref")
 
 
 def build_all(verbose=False, jobs=None):
@@ -258,6 +260,8 @@ def build_all(verbose=False, jobs=None):
                         instr, block=block, waves_per_eu=w, streams_per_lane=spl)[0]
     for name in REFILLED:
         srcs[name + "@refill"] = generate_source(instruments()[name](), refill=True)[0]
+    for name in UNCOMPACTED:
+        srcs[name + "@plain"] = generate_source(instruments()[name](), compact=False)[0]
     unique = sorted(set(srcs.values()))
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:

@@ -33,6 +33,8 @@ WORKLOADS = {
     "c2s": ("guide transport + isotropic sphere sample (r 1 cm) -> I(Q,E) 120x100 + PSD 250x250", int(1e8)),
     "c1": ("Source_simple (E 60+-10 meV) -> isotropic sphere (r 1 cm, 5 b / 2 b) -> I(Q,E) 120x100 + "
            "PSD 250x250", int(1e7)),
+    "c3": ("Source_simple (E 60+-10 meV) -> hollow cylinder (r 1.0/1.5 cm, h 5 cm), S(Q,E) grid 200x200, "
+           "multiple scattering -> 400 tubes x 128 pixels x 500 TOF bins", int(1e8)),
 }
 
 
@@ -62,13 +64,14 @@ def oracle_flops_per_neutron(instr, n=1000000, seed=0):
             "neutrons": int(processed)}
 
 
-def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0):
-    """Oracle on this host's cores over a bounded leading part of the workload
-    (whole streams of the reference partition)."""
+def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0, cores=None):
+    """Oracle on this host's cores (or `cores` threads) over a bounded leading
+    part of the workload (whole streams of the reference partition)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from acc_amd.instrument import calcTransformations
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    if cores is None:
+        cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     descs = [c.mcv_desc() for c in instr.components]
     off, rot = calcTransformations(instr)
     nblocks, nthreads, npt = pyoracle.partition(ncount, ntotalthreads, tpb)
@@ -185,6 +188,58 @@ def copy_kernel(nbytes=1 << 31, reps=3):
     return 0
 
 
+CONFIG_TABLE = (("c1", int(1e7)), ("c2", int(1e8)), ("c2s", int(1e8)), ("c3", int(1e8)))
+
+
+def config_table(steps=3, cpu_seconds=5.0, falg_neutrons=100000):
+    """The survey's configurations side by side: fused-kernel time (HIP events)
+    and wall time per run at the reference partition, neutrons/s, F_alg and
+    the FP64 fraction, and the CPU oracle on all host cores and on one.
+    c2 and c2s run at their autotuned shape, c1 and c3 at the default one."""
+    from acc_amd import configs
+    from acc_amd._abi import MCV_FLAG_GATE_DEAD
+    from acc_amd.autotune import tuned_handle
+    from acc_amd.run_script import get_handle
+    rows = {}
+    for name, ncount in CONFIG_TABLE:
+        print("[configs] %s %d neutrons" % (name, ncount), file=sys.stderr, flush=True)
+        instr = configs.BUILDERS[name]()
+        if name in ("c2", "c2s"):
+            h, best = tuned_handle(instr, ncount, use_cache=False)
+            shape = "%s/%s/%s" % (best["block"], best["waves_per_eu"], best["streams_per_lane"])
+        else:
+            h, shape = get_handle(instr), "default"
+        h.clear()
+        h.run(ncount, 0, 0, 0, MCV_FLAG_GATE_DEAD)
+        kms, walls = [], []
+        for k in range(steps):
+            h.clear()
+            t = time.perf_counter()
+            kms.append(h.run(ncount, k, 0, 0, MCV_FLAG_GATE_DEAD)["ms_kernel"])
+            walls.append((time.perf_counter() - t) * 1e3)
+        fl = oracle_flops_per_neutron(configs.BUILDERS[name](), n=falg_neutrons)
+        k_avg, w_avg = sum(kms) / steps, sum(walls) / steps
+        cpu_all = cpu_baseline(configs.BUILDERS[name](), ncount, seconds=cpu_seconds)
+        cpu_one = cpu_baseline(configs.BUILDERS[name](), ncount, seconds=cpu_seconds, cores=1)
+        rows[name] = {"ncount": ncount, "launch_shape": shape, "ms_kernel": k_avg, "ms_wall": w_avg,
+                      "neutrons_per_s_kernel": ncount / (k_avg * 1e-3), "neutrons_per_s_wall": ncount / (w_avg * 1e-3),
+                      "falg_ops_per_neutron": fl["F_alg"], "draws_per_neutron": fl["draws"],
+                      "fp64_frac": fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                      "cpu_all_cores": cpu_all, "cpu_one_core": cpu_one}
+    return {"configs": rows, "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def traffic_from_profiles(workload, shape):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE for
     this workload's kernel at this launch shape (the shapes' spills differ), or
@@ -216,9 +271,14 @@ def main():
     ap.add_argument("--shape", default=None, help="fixed launch shape BLOCK/WAVES[/STREAMS_PER_LANE] (e.g. 256/3) instead of tuning")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration) and exit")
+    ap.add_argument("--configs", action="store_true",
+                    help="only print the c1 / c2 / c2s / c3 table (GPU and CPU oracle rates) and exit")
     a = ap.parse_args()
     if a.copy_kernel:
         return copy_kernel()
+    if a.configs:
+        print(json.dumps(config_table()))
+        return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
               "(multi-GPU sharding by stream ranges is described in DESIGN.md); use --gpus 1", file=sys.stderr)

@@ -27,6 +27,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 #include <algorithm>
@@ -1325,6 +1326,35 @@ static void aperture(double* n, const double* P, bool beamstop) {
   n[8] = t;
 }
 
+// One monitor event: (count, sum p, sum p^2) into bin b of an nb-bin block.
+// Runs whose per-thread histogram copies would be too large share one
+// histogram between threads; their adds are atomic.
+static thread_local bool g_shared_hist = false;
+static inline void hist_event(double* out, int64_t nb, int64_t b, double p) {
+  const double p2 = p * p;
+  if (g_shared_hist) {
+#ifdef ORC_FLOP_BUILD  // the counting type is not an atomic operand
+#pragma omp critical(orc_hist)
+    {
+      out[b] += 1;
+      out[nb + b] += p;
+      out[2 * nb + b] += p2;
+    }
+#else
+#pragma omp atomic
+    out[b] += 1;
+#pragma omp atomic
+    out[nb + b] += p;
+#pragma omp atomic
+    out[2 * nb + b] += p2;
+#endif
+  } else {
+    out[b] += 1;
+    out[nb + b] += p;
+    out[2 * nb + b] += p2;
+  }
+}
+
 // ---------------------------------------------------------------- monitors (R14-R17)
 // acc/components/monitors/psd_monitor.py:79-96
 static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny, double* out) {
@@ -1340,9 +1370,7 @@ static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny
   if (ix >= nx) ix = nx - 1;  // the reference writes past the row here (no bounds check)
   if (iy >= ny) iy = ny - 1;
   int64_t nb = nx * ny, b = ix * ny + iy;
-  out[b] += 1;
-  out[nb + b] += p;
-  out[2 * nb + b] += p * p;
+  hist_event(out, nb, b, p);
 }
 
 // acc/components/monitors/posdiv_monitor.py:70-95 (NaN rejected and indices
@@ -1362,9 +1390,7 @@ static void posdiv_monitor(const double* n, const double* P, int64_t npos, int64
   ix = std::min<int64_t>(std::max<int64_t>(ix, 0), npos - 1);
   idiv = std::min<int64_t>(std::max<int64_t>(idiv, 0), ndiv - 1);
   int64_t nb = npos * ndiv, b = ix * ndiv + idiv;
-  out[b] += 1;
-  out[nb + b] += p;
-  out[2 * nb + b] += p * p;
+  hist_event(out, nb, b, p);
 }
 
 // acc/components/monitors/psd_monitor_4pi.py:59-81 with
@@ -1392,9 +1418,7 @@ static void psd4pi_monitor(double* n, const double* P, int64_t nphi, int64_t nth
   itheta = std::min<int64_t>(std::max<int64_t>(itheta, 0), ntheta - 1);
   double p = n[9];
   int64_t nb = nphi * ntheta, b = iphi * ntheta + itheta;
-  out[b] += 1;
-  out[nb + b] += p;
-  out[2 * nb + b] += p * p;
+  hist_event(out, nb, b, p);
 }
 
 // acc/components/monitors/wavelength_monitor.py:149-171
@@ -1411,9 +1435,7 @@ static void wavelength_monitor(const double* n, const double* P, int64_t nchan, 
   if (!(L > Lmin && L < Lmax)) return;
   int64_t iL = (int64_t)std::floor((L - Lmin) / (Lmax - Lmin) * nchan);
   if (iL >= nchan) iL = nchan - 1;
-  out[iL] += 1;
-  out[nchan + iL] += p;
-  out[2 * nchan + iL] += p * p;
+  hist_event(out, nchan, iL, p);
 }
 
 // acc/components/monitors/iqe_monitor.py:251-284
@@ -1440,9 +1462,7 @@ static void iqe_monitor(const double* n, const double* P, int64_t nQ, int64_t nE
       if (iE >= nE) iE = nE - 1;
       double p = n[9];
       int64_t nb = nQ * nE, b = iQ * nE + iE;
-      out[b] += 1;
-      out[nb + b] += p;
-      out[2 * nb + b] += p * p;
+      hist_event(out, nb, b, p);
     }
   }
 }
@@ -1475,9 +1495,7 @@ static void tube_detector(const double* n, const double* P, int64_t ntubes, int6
   if (ib >= ntof) ib = ntof - 1;
   double p = n[9];
   int64_t nb = ntubes * npix * ntof, bin = (it * npix + ip) * ntof + ib;
-  out[bin] += 1;
-  out[nb + bin] += p;
-  out[2 * nb + bin] += p * p;
+  hist_event(out, nb, bin, p);
 }
 
 // ---------------------------------------------------------------- chain
@@ -1777,14 +1795,20 @@ int64_t orc_run(const mcv_comp_desc* comps, int ncomp, const double* offsets, co
   init_states(seed, tid1, states.data());
   int nt = nthreads_omp > 0 ? nthreads_omp : omp_get_max_threads();
   int64_t processed = 0;
-  std::vector<std::vector<double>> hists(nt, std::vector<double>(C.hist_total, 0.0));
+  // per-thread histograms, or one shared (atomic adds) when the copies would
+  // exceed ORC_SHARED_HIST_MB (default 512)
+  const char* lim = std::getenv("ORC_SHARED_HIST_MB");
+  const double limit_mb = lim && *lim ? std::atof(lim) : 512.0;
+  const bool shared = nt > 1 && (double)C.hist_total * 8.0 * nt > limit_mb * (1 << 20);
+  std::vector<std::vector<double>> hists(shared ? 0 : nt, std::vector<double>(C.hist_total, 0.0));
   int maxms = 1;
   for (auto& c : C.comps)
     if (c.type == MCV_SAMPLE_HMS) maxms = std::max(maxms, c.sample.max_scattered);
 #pragma omp parallel num_threads(nt) reduction(+ : processed)
   {
     int me = omp_get_thread_num();
-    double* h = hists[me].data();
+    g_shared_hist = shared;
+    double* h = shared ? hist : hists[me].data();
     std::vector<double> msbuf(10 * maxms);
 #pragma omp for schedule(dynamic, 64)
     for (int64_t tid = tid0; tid < tid1; ++tid) {
@@ -1803,8 +1827,9 @@ int64_t orc_run(const mcv_comp_desc* comps, int ncomp, const double* offsets, co
         }
       }
     }
+    g_shared_hist = false;
   }
-  for (int t = 0; t < nt; ++t)
+  for (size_t t = 0; t < hists.size(); ++t)
     for (int64_t k = 0; k < C.hist_total; ++k) hist[k] += hists[t][k];
   return processed;
 }
@@ -1851,8 +1876,9 @@ int64_t orc_process(const mcv_comp_desc* comps, int ncomp, const double* offsets
         }
       }
     }
+    g_shared_hist = false;
   }
-  for (int t = 0; t < nt; ++t)
+  for (size_t t = 0; t < hists.size(); ++t)
     for (int64_t k = 0; k < C.hist_total; ++k) hist[k] += hists[t][k];
   return processed;
 }

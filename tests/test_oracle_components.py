@@ -276,3 +276,29 @@ def test_psd4pi_monitor_closed_form():
     out, h = _one(d, n)
     np.testing.assert_array_equal(out, n)
     assert h[0].sum() == 0
+
+
+def test_shared_histogram_mode_matches_private(monkeypatch):
+    """The oracle's threads share one histogram (atomic adds) when their
+    private copies would be too large: the counts are identical and the sums
+    agree to rounding with the private-copy run and with one thread."""
+    from acc_amd import configs
+    from acc_amd.instrument import calcTransformations
+    instr = configs.c1()
+    descs = [c.mcv_desc() for c in instr.components]
+    off, rot = calcTransformations(instr)
+    N = 200000
+    one = orc.run(descs, off, rot, N, 0, ntotalthreads=4096, tpb=512, gate=True, nthreads=1)[0]
+    private = orc.run(descs, off, rot, N, 0, ntotalthreads=4096, tpb=512, gate=True, nthreads=4)[0]
+    monkeypatch.setenv("ORC_SHARED_HIST_MB", "0")
+    shared = orc.run(descs, off, rot, N, 0, ntotalthreads=4096, tpb=512, gate=True, nthreads=4)[0]
+    seen = 0
+    for a, b, c in zip(one, private, shared):
+        if a is None:
+            continue
+        seen += a[0].sum()
+        np.testing.assert_array_equal(b[0], a[0])
+        np.testing.assert_array_equal(c[0], a[0])
+        np.testing.assert_allclose(b[1:], a[1:], rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(c[1:], a[1:], rtol=1e-12, atol=1e-300)
+    assert seen > 0
