@@ -64,8 +64,12 @@ __device__ __forceinline__ void dump_neutron(const LaunchArgs& A, int64_t idx, c
 // ------------------------------------------------------------------ monitor histograms
 // Small monitor histograms live in LDS for the lifetime of a block of streams
 // (LDS FP64 atomics), and are added to the global histograms with one
-// device-scope atomic per non-empty bin when the block is done; the rest go
-// straight to global memory.
+// device-scope atomic per non-empty bin when the block is done.  Larger ones
+// are replicated in global memory when that fits a budget: adds to one bin
+// serialise at the memory-side atomic unit, and a direct beam puts thousands
+// of adds on each of a few thousand bins, so the workgroups spread theirs
+// over the replicas (the runtime folds them afterwards); the rest go straight
+// to the histogram.
 extern __shared__ double mcv_lds_hist[];
 
 __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_t nbins, int64_t b, double p) {
@@ -75,6 +79,9 @@ __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_
     atomicAdd(L + b, 1.0);
     atomicAdd(L + nbins + b, p);
     atomicAdd(L + 2 * nbins + b, p * p);
+  } else if (A.roff[comp] >= 0) {
+    const int64_t r = (int64_t)(blockIdx.x & (unsigned)A.rmask[comp]);
+    hist_add3(A.R + A.roff[comp] + r * 3 * nbins, nbins, b, p);
   } else {
     hist_add3(A.H + A.hoff[comp], nbins, b, p);
   }

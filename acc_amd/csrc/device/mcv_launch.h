@@ -35,6 +35,12 @@ struct LaunchArgs {
   int32_t lds_nseg, lds_total;
   int32_t lds_loff[MCV_LDS_SEG], lds_len[MCV_LDS_SEG];
   int64_t lds_hoff[MCV_LDS_SEG];
+  // Monitor histograms replicated in global memory: monitor i adds into
+  // replica (workgroup & rmask[i]) of its block at R + roff[i] (roff[i] = -1:
+  // straight into H); the runtime folds the replicas into H after the launch.
+  double* R;
+  int64_t roff[MCV_MAX_COMP];
+  int32_t rmask[MCV_MAX_COMP];
   // buffered mode: neutron i of the run is row i of io (N, 10), updated in
   // place; a multiple-scattering sample writes its io_m outputs per neutron to
   // io_out rows [i*io_m, (i+1)*io_m)

@@ -404,9 +404,23 @@ int64_t lds_hist_budget() {
   return v < 0 ? 0 : std::min<long long>(v, 8192);
 }
 
+// Replicas per global monitor histogram (MCV_HIST_REPLICAS from the
+// environment, rounded down to a power of two; 0 or 1 turns replication off),
+// default 64, each monitor's replicas capped at 64 MB.
+int hist_replicas() {
+  const char* e = std::getenv("MCV_HIST_REPLICAS");
+  long long v = (e && *e) ? std::atoll(e) : 64;
+  if (v < 2) return 1;
+  int r = 1;
+  while ((long long)r * 2 <= std::min<long long>(v, 1024)) r *= 2;
+  return r;
+}
+constexpr int64_t kReplicaBytes = int64_t(64) << 20;
+
 int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
                  mcv::LaunchArgs& args, std::vector<double>& B, std::vector<int64_t>& nbins, int64_t& hist_total,
-                 std::vector<int>& types, std::vector<int>& nparams) {
+                 std::vector<int>& types, std::vector<int>& nparams, bool replicate = false,
+                 int64_t* rep_total = nullptr) {
   std::vector<Packed> packed(ncomp);
   for (int i = 0; i < ncomp; ++i) {
     int rc = pack(comps[i], i, packed[i]);
@@ -453,6 +467,25 @@ int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offset
     args.lds_nseg += 1;
     args.lds_total += (int32_t)len;
   }
+  // the other monitors: replicated in global memory while the copies fit
+  args.R = nullptr;
+  int64_t rtot = 0;
+  const int reps = replicate ? hist_replicas() : 1;
+  for (int i = 0; i < MCV_MAX_COMP; ++i) {
+    args.roff[i] = -1;
+    args.rmask[i] = 0;
+  }
+  for (int i = 0; i < ncomp && reps > 1; ++i) {
+    const int64_t len = 3 * packed[i].nbins;
+    if (len <= 0 || args.loff[i] >= 0) continue;
+    int r = reps;
+    while (r > 1 && len * (int64_t)sizeof(double) * r > kReplicaBytes) r /= 2;
+    if (r < 2) continue;
+    args.roff[i] = rtot;
+    args.rmask[i] = r - 1;
+    rtot += r * len;
+  }
+  if (rep_total) *rep_total = rtot;
   args.ncomp = ncomp;
   return MCV_OK;
 }
@@ -460,6 +493,18 @@ int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offset
 __global__ void mcv_zero_u64(unsigned long long* p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+// H[j] += sum over replicas r of R[r * len + j], replicas zeroed for the next launch
+__global__ void mcv_fold_replicas(double* __restrict__ H, double* __restrict__ R, int64_t len, int reps) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < len; j += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int r = 0; r < reps; ++r) {
+      s += R[r * len + j];
+      R[r * len + j] = 0;
+    }
+    if (s != 0) H[j] += s;
+  }
 }
 
 __global__ void mcv_zero_f64(double* p, int64_t n) {
@@ -481,6 +526,8 @@ struct mcv_instr {
   int64_t hist_total = 0;
   double* d_blob = nullptr;
   double* d_hist = nullptr;
+  double* d_rep = nullptr;  // monitor replicas (zero between launches)
+  int64_t rep_total = 0;
   uint64_t* d_rng = nullptr;
   int64_t rng_cap = 0;
   unsigned long long* d_counter = nullptr;
@@ -550,6 +597,7 @@ void release(mcv_instr* h) {
   if (!h) return;
   if (h->d_blob) (void)hipFree(h->d_blob);
   if (h->d_hist) (void)hipFree(h->d_hist);
+  if (h->d_rep) (void)hipFree(h->d_rep);
   if (h->d_rng) (void)hipFree(h->d_rng);
   if (h->d_counter) (void)hipFree(h->d_counter);
   if (h->d_dump) (void)hipFree(h->d_dump);
@@ -580,7 +628,7 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   *out = h;
   h->ncomp = ncomp;
   int rc = build_layout(comps, ncomp, offsets, rotmats, h->args, h->blob, h->nbins, h->hist_total, h->types,
-                        h->nparams);
+                        h->nparams, true, &h->rep_total);
   if (rc) return rc;
   std::vector<double>& B = h->blob;
 
@@ -595,6 +643,10 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   HIPCHECK(hipMemcpy(h->d_blob, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMalloc(&h->d_hist, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
   HIPCHECK(hipMemset(h->d_hist, 0, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
+  if (h->rep_total > 0) {
+    HIPCHECK(hipMalloc(&h->d_rep, h->rep_total * sizeof(double)));
+    HIPCHECK(hipMemset(h->d_rep, 0, h->rep_total * sizeof(double)));
+  }
   HIPCHECK(hipMalloc(&h->d_counter, sizeof(unsigned long long)));
 
   hipError_t e = hipModuleLoad(&h->mod, code_object);
@@ -612,6 +664,19 @@ void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks,
   nblocks = (ntot + tpb - 1) / tpb;
   nthreads = nblocks * tpb;
   npt = nthreads > 0 ? (int64_t)((N + (uint64_t)nthreads - 1) / (uint64_t)nthreads) : 0;
+}
+
+// Adds the replicated monitors' copies into their histograms (and zeroes them).
+int fold_replicas(mcv_instr* h) {
+  for (int i = 0; i < h->ncomp; ++i) {
+    if (h->args.roff[i] < 0) continue;
+    const int64_t len = 3 * h->nbins[i];
+    const int64_t grid = std::min<int64_t>((len + 255) / 256, 4096);
+    hipLaunchKernelGGL(mcv_fold_replicas, dim3((unsigned)grid), dim3(256), 0, h->stream, h->d_hist + h->args.hoff[i],
+                       h->d_rep + h->args.roff[i], len, h->args.rmask[i] + 1);
+    HIPCHECK(hipGetLastError());
+  }
+  return MCV_OK;
 }
 
 }  // namespace
@@ -693,6 +758,7 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   mcv::LaunchArgs A = h->args;
   A.P = h->d_blob;
   A.H = h->d_hist;
+  A.R = h->d_rep;
   A.rng = h->d_rng;
   A.counter = h->d_counter;
   A.dump = h->d_dump;
@@ -710,6 +776,8 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, (unsigned)(A.lds_total * sizeof(double)),
                                  h->stream, kargs, nullptr));
+  rc = fold_replicas(h);
+  if (rc) return rc;
   HIPCHECK(hipEventRecord(h->ev[2], h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   unsigned long long processed = 0;
@@ -939,6 +1007,7 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
   mcv::LaunchArgs A = h->args;
   A.P = h->d_blob;
   A.H = h->d_hist;
+  A.R = h->d_rep;
   A.rng = h->d_rng;
   A.counter = h->d_counter;
   A.dump = nullptr;
@@ -960,6 +1029,10 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->proc_fn, (unsigned)((nthreads + bs - 1) / bs), 1, 1, bs, 1, 1,
                                  (unsigned)(A.lds_total * sizeof(double)), h->stream, kargs, nullptr));
+  if (m == 0) {
+    rc = fold_replicas(h);
+    if (rc) return rc;
+  }
   HIPCHECK(hipEventRecord(h->ev[2], h->stream));
   HIPCHECK(hipStreamSynchronize(h->stream));
   unsigned long long processed = 0;

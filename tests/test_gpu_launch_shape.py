@@ -116,3 +116,36 @@ def test_ms_compaction_invariance(name):
     for i in h0:
         np.testing.assert_array_equal(h1[i][0], h0[i][0])
         np.testing.assert_allclose(h1[i][1:], h0[i][1:], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s"])
+def test_replicated_histograms(name, monkeypatch):
+    """Monitors whose adds are spread over replicas (and folded after the
+    launch) give the histograms of unreplicated adds, on every run of a
+    handle (the fold leaves the replicas zeroed for the next one)."""
+    from acc_amd._abi import Handle
+    from acc_amd.fuser import fuse
+    from acc_amd.instrument import calcTransformations
+    print(f"\n[launch] {name} replicated histograms", flush=True)
+    instr = prebuilt.instruments()[name]()
+    off, rot = calcTransformations(instr)
+    p, k = fuse(instr)
+    N, ntot = 400000, 20000
+    res = {}
+    for reps in ("1", "64"):
+        monkeypatch.setenv("MCV_HIST_REPLICAS", reps)
+        h = Handle([c.mcv_desc() for c in instr.components], off, rot, p, k)
+        try:
+            runs = []
+            for seed in (1, 2):
+                h.clear()
+                assert h.run(N, seed, ntot, 512, MCV_FLAG_GATE_DEAD)["processed"] == N
+                runs.append({i: h.read_monitor(i) for i in range(len(instr.components)) if h.nbins[i] > 0})
+            res[reps] = runs
+        finally:
+            h.close()
+    for a, b in zip(res["1"], res["64"]):
+        for i in a:
+            assert a[i][0].sum() > 0
+            np.testing.assert_array_equal(b[i][0], a[i][0])
+            np.testing.assert_allclose(b[i][1:], a[i][1:], rtol=1e-12, atol=1e-300)
