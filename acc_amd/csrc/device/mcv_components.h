@@ -89,8 +89,8 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
 // execute whenever any of its lanes takes the branch).  The winning wall's
 // time is divided afterwards from the same operands as in the reference, so
 // a path is bit for bit the reference's unless two times agree to the last
-// bits (a corner hit).  A zero or non-finite denominator product (vz = 0, an
-// overflow) sets `odd`; the caller then redoes the chain with divisions.
+// bits (a corner hit).  A zero or NaN denominator (vz = 0) sets `odd`; the
+// caller then redoes the chain with divisions.
 struct ExitChoice {
   double n, d;  // the earliest exit time so far is n / d
   int i;        // 0: the segment (guide) end; 1-4: a wall
@@ -98,11 +98,12 @@ struct ExitChoice {
 };
 
 __device__ __forceinline__ void exit_candidate(ExitChoice& e, double n, double d, int k) {
-  const double s = d * e.d;
+  // d < 0, so d * e.d has the sign opposite to e.d's: n/d < e.n/e.d is
+  // n * e.d > e.n * d when e.d > 0, and n * e.d < e.n * d when e.d < 0
   const double l = n * e.d, r = e.n * d;
   const bool towards = d < 0;
-  const bool lt = (s > 0 && l < r) || (s < 0 && l > r);
-  e.odd = e.odd || (towards && !(s > 0 || s < 0));
+  const bool lt = e.d > 0 ? l > r : l < r;
+  e.odd = e.odd || (towards && !(e.d > 0 || e.d < 0));
   const bool take = towards && lt;
   e.n = take ? n : e.n;
   e.d = take ? d : e.d;
@@ -239,10 +240,13 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // ------------------------------------------------------------------ Guide_tapering
 // acc/components/optics/guide_tapering.py:141-291.
 // P = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W, nseg (as double);
-// D = 16 x nseg table: w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf from
-// the component, then rows the host derives per segment: l_seg^2 + ww^2 and its
-// sqrt, l_seg^2 + hh^2 and its sqrt (the reference's nlen2 and sqrt(nlen2)),
-// whalf * l_seg and hhalf * l_seg (the reference's products in cv1 and ch1).
+// D = one 16-double record per segment, packed by the runtime from the
+// component's 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf,
+// lhhalf): [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf *
+// l_seg (the reference's products in cv1 and ch1), [6] l_seg^2 + ww^2 and [7]
+// its sqrt, [8] l_seg^2 + hh^2 and [9] its sqrt (the reference's nlen2 and
+// sqrt(nlen2)), [10] h1, [11] whalf.  A segment entrance reads its six values
+// from one 128-byte record instead of six rows nseg apart.
 // The reference's two nested loops (segments, then bounces inside a segment)
 // run here as one event loop: each pass computes the next wall or segment-end
 // time of the current segment, then either reflects or moves on to the next
@@ -298,16 +302,15 @@ __device__ __forceinline__ void taper_init(TaperState& s, const Neutron& n, cons
 template <class T>
 __device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __restrict__ P,
                                               const double* __restrict__ D, T& tr) {
-  const int nseg = (int)P[9];
   const bool vert = s.pax == 0;
-  const double q = V2K * (-2.0) * s.pvdn / D[(vert ? 11 : 13) * nseg + s.pseg];
+  const double* __restrict__ rec = D + 16 * s.pseg;
+  const double q = V2K * (-2.0) * s.pvdn / rec[vert ? 7 : 9];
   const double mxy = vert ? P[6] : P[7];
   double m = mxy, dm = 1.0;
-  if (m <= 0.0) {
-    const double* __restrict__ w1 = D;
-    const double* __restrict__ h1 = D + 2 * nseg;
-    const double ratio = vert ? w1[0] / w1[s.pseg] : h1[0] / h1[s.pseg];
-    m = vert ? fabs(mxy * w1[0] / w1[s.pseg]) : fabs(mxy * h1[0] / h1[s.pseg]);
+  if (m <= 0.0) {  // m = |mxy * w1[0] / w1[seg]| (or h1)
+    const double a0 = vert ? D[0] : D[10], as = vert ? rec[0] : rec[10];
+    const double ratio = a0 / as;
+    m = fabs(mxy * a0 / as);
     dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
   }
   const double Qc = vert ? P[2] : P[3], alpha = vert ? P[4] : P[5];
@@ -329,12 +332,13 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   const double l_seg = P[0];
   const int nseg = (int)P[9];
   if (s.enter) {  // segment entrance: guide_tapering.py:170-190
-    const double w1s = D[s.seg], hhs = D[7 * nseg + s.seg];
+    const double* __restrict__ rec = D + 16 * s.seg;
+    const double w1s = rec[0], hhs = rec[1];
     s.zr = (double)s.seg * l_seg;
-    s.wws = D[4 * nseg + s.seg];
-    s.hhh = D[5 * nseg + s.seg];
-    s.lwh = D[14 * nseg + s.seg];
-    s.lhh = D[15 * nseg + s.seg];
+    s.wws = rec[2];
+    s.hhh = rec[3];
+    s.lwh = rec[4];
+    s.lhh = rec[5];
     const double dt = (s.zr - s.z) / s.vz;
     s.x += s.vx * dt;
     s.y += s.vy * dt;
@@ -389,7 +393,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
       const bool vert = i <= 2;
       const double wh = vert ? wws : hhh;
       const double vdn = e.d;
-      const double d = 2.0 * vdn / D[(vert ? 10 : 12) * nseg + s.seg];
+      const double d = 2.0 * vdn / D[16 * s.seg + (vert ? 6 : 8)];
       const double dl = d * l_seg;
       s.vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
       s.vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;

@@ -230,21 +230,36 @@ void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, .
   out[3] = std::sqrt(out[2]);
 }
 
-// Tapered guide, per segment (D: 16 x nseg, rows 0-9 given): the mirror norms
-// as above, and whalf * l_seg, hhalf * l_seg (the products of cv1 and ch1,
-// guide_tapering.py:202-204).
-void taper_norms(double l_seg, int64_t nseg, double* D) {
+// Tapered guide: one 16-double record per segment from the component's
+// 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf):
+// [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf * l_seg
+// (the products of cv1 and ch1, guide_tapering.py:202-204), [6] nlen2 and [7]
+// its sqrt for the vertical mirrors, [8] and [9] for the horizontal ones,
+// [10] h1, [11] whalf.
+constexpr int64_t kTaperRec = 16;
+
+void taper_record(double l_seg, double w1, double h1, double ww, double hh, double whalf, double hhalf, double* r) {
   const double l2 = l_seg * l_seg;
-  for (int64_t k = 0; k < nseg; ++k) {
-    const double ww = D[4 * nseg + k], hh = D[5 * nseg + k];
-    const double w2 = ww * ww, h2 = hh * hh;
-    D[10 * nseg + k] = l2 + w2;
-    D[11 * nseg + k] = std::sqrt(D[10 * nseg + k]);
-    D[12 * nseg + k] = l2 + h2;
-    D[13 * nseg + k] = std::sqrt(D[12 * nseg + k]);
-    D[14 * nseg + k] = D[6 * nseg + k] * l_seg;
-    D[15 * nseg + k] = D[7 * nseg + k] * l_seg;
-  }
+  const double w2 = ww * ww, h2 = hh * hh;
+  r[0] = w1;
+  r[1] = hhalf;
+  r[2] = ww;
+  r[3] = hh;
+  r[4] = whalf * l_seg;
+  r[5] = hhalf * l_seg;
+  r[6] = l2 + w2;
+  r[7] = std::sqrt(r[6]);
+  r[8] = l2 + h2;
+  r[9] = std::sqrt(r[8]);
+  r[10] = h1;
+  r[11] = whalf;
+  for (int k = 12; k < kTaperRec; ++k) r[k] = 0;
+}
+
+void taper_records(double l_seg, int64_t nseg, const double* T, double* D) {  // T: 10 x nseg
+  for (int64_t k = 0; k < nseg; ++k)
+    taper_record(l_seg, T[k], T[2 * nseg + k], T[4 * nseg + k], T[5 * nseg + k], T[6 * nseg + k], T[7 * nseg + k],
+                 D + kTaperRec * k);
 }
 
 void cross3(const double* a, const double* b, double* o) {
@@ -330,9 +345,8 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
                     (long long)nseg, (long long)d.n_data);
       copyp(9);
       out.params.push_back((double)nseg);
-      out.data.assign(d.data, d.data + d.n_data);
-      out.data.resize(16 * nseg);
-      taper_norms(d.params[0], nseg, out.data.data());
+      out.data.assign(kTaperRec * nseg, 0.0);
+      taper_records(d.params[0], nseg, d.data, out.data.data());
       return MCV_OK;
     }
     case MCV_SAMPLE_HSS:
@@ -723,9 +737,11 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
   } else if (h->types[comp] == MCV_GUIDE_TAPERING) {
     const int64_t nseg = (int64_t)h->blob[off + 9];
     double* D = h->blob.data() + h->args.doff[comp];
-    taper_norms(params[0], nseg, D);
-    HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp] + 10 * nseg, D + 10 * nseg, sizeof(double) * 6 * nseg,
-                       hipMemcpyHostToDevice));
+    for (int64_t k = 0; k < nseg; ++k) {
+      double* r = D + kTaperRec * k;
+      taper_record(params[0], r[0], r[10], r[2], r[3], r[11], r[1], r);
+    }
+    HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp], D, sizeof(double) * kTaperRec * nseg, hipMemcpyHostToDevice));
   }
   return MCV_OK;
 }
