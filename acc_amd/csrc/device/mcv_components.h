@@ -243,10 +243,11 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // D = one 16-double record per segment, packed by the runtime from the
 // component's 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf,
 // lhhalf): [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf *
-// l_seg (the reference's products in cv1 and ch1), [6] l_seg^2 + ww^2 and [7]
-// its sqrt, [8] l_seg^2 + hh^2 and [9] its sqrt (the reference's nlen2 and
-// sqrt(nlen2)), [10] h1, [11] whalf.  A segment entrance reads its six values
-// from one 128-byte record instead of six rows nseg apart.
+// l_seg (the reference's products in cv1 and ch1), [6] zr = seg * l_seg, [7]
+// zr + l_seg, [8] l_seg^2 + ww^2 and [9] its sqrt, [10] l_seg^2 + hh^2 and
+// [11] its sqrt (the reference's nlen2 and sqrt(nlen2)), [12] h1, [13] whalf.
+// A segment entrance reads its eight values from one 128-byte record instead
+// of rows nseg apart.
 // The reference's two nested loops (segments, then bounces inside a segment)
 // run here as one event loop: each pass computes the next wall or segment-end
 // time of the current segment, then either reflects or moves on to the next
@@ -272,7 +273,7 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 
 struct TaperState {
   double x, y, z, vx, vy, vz, t, prob;
-  double zr, wws, hhh, lwh, lhh, hadj;  // current segment (entrance z, slopes, l_seg x half sizes, offset)
+  double zr, zend, wws, hhh, lwh, lhh, hadj;  // current segment (entrance and end z, slopes, l_seg x half sizes, offset)
   double pvdn;                          // pending reflection: v.n of its wall
   int seg, nb;
   int pseg, pax;  // pending reflection's segment and axis (0 vertical, 1 horizontal; -1 none)
@@ -289,7 +290,7 @@ __device__ __forceinline__ void taper_init(TaperState& s, const Neutron& n, cons
   s.t += dt;
   s.seg = 0;
   s.nb = 0;
-  s.zr = s.wws = s.hhh = s.lwh = s.lhh = s.hadj = 0;
+  s.zr = s.zend = s.wws = s.hhh = s.lwh = s.lhh = s.hadj = 0;
   s.pvdn = 0;
   s.pseg = 0;
   s.pax = -1;
@@ -304,11 +305,11 @@ __device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __res
                                               const double* __restrict__ D, T& tr) {
   const bool vert = s.pax == 0;
   const double* __restrict__ rec = D + 16 * s.pseg;
-  const double q = V2K * (-2.0) * s.pvdn / rec[vert ? 7 : 9];
+  const double q = V2K * (-2.0) * s.pvdn / rec[vert ? 9 : 11];
   const double mxy = vert ? P[6] : P[7];
   double m = mxy, dm = 1.0;
   if (m <= 0.0) {  // m = |mxy * w1[0] / w1[seg]| (or h1)
-    const double a0 = vert ? D[0] : D[10], as = vert ? rec[0] : rec[10];
+    const double a0 = vert ? D[0] : D[12], as = vert ? rec[0] : rec[12];
     const double ratio = a0 / as;
     m = fabs(mxy * a0 / as);
     dm = mxy * ratio < 0 ? -ratio : ratio;  // d|mxy * ratio| / d mxy
@@ -334,28 +335,28 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   if (s.enter) {  // segment entrance: guide_tapering.py:170-190
     const double* __restrict__ rec = D + 16 * s.seg;
     const double w1s = rec[0], hhs = rec[1];
-    s.zr = (double)s.seg * l_seg;
     s.wws = rec[2];
     s.hhh = rec[3];
     s.lwh = rec[4];
     s.lhh = rec[5];
+    s.zr = rec[6];
+    s.zend = rec[7];
     const double dt = (s.zr - s.z) / s.vz;
     s.x += s.vx * dt;
     s.y += s.vy * dt;
     s.z += s.vz * dt;
-    if (s.x <= -0.5 * w1s || s.x >= 0.5 * w1s || s.y <= -hhs || s.y >= hhs) {
+    // the opening (p = 0 outside), then the channel shift; both tests are
+    // evaluated without short-circuit branches.  After the shift x is in
+    // [0, w1s] (rounding keeps x > -w1s/2 from going negative), so floor(x /
+    // w1s) is 0 unless the quotient rounds up to 1: divide only then
+    const bool outside = (s.x <= -0.5 * w1s) | (s.x >= 0.5 * w1s) | (s.y <= -hhs) | (s.y >= hhs);
+    const double x1 = s.x + 0.5 * w1s;
+    const double edge = x1 < 0.75 * w1s ? 0.0 : floor(x1 / w1s) * w1s;
+    if (outside | (x1 - edge > w1s)) {
       s.died = true;
       return true;
     }
-    s.x += 0.5 * w1s;
-    // x is now in [0, w1s] (rounding keeps x > -w1s/2 from going negative), so
-    // floor(x / w1s) is 0 unless the quotient rounds up to 1: divide only then
-    const double edge = s.x < 0.75 * w1s ? 0.0 : floor(s.x / w1s) * w1s;
-    if (s.x - edge > w1s) {
-      s.died = true;
-      return true;
-    }
-    s.x -= (edge + 0.5 * w1s);
+    s.x = x1 - (edge + 0.5 * w1s);
     s.hadj = edge + (0.5 * w1s) - 0.5 * w1s;
     s.nb = 0;
     s.enter = false;
@@ -372,7 +373,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   const double ch1 = -s.lhh - (z - zr) * hhh, ch2 = y * l_seg;
   ExitChoice e{0.0, 0.0, 0, false};
   if (!seg_end)
-    e = earliest_exit(zr + l_seg - z, vz, cv1 - cv2, vdotn_v1, cv1 + cv2, vdotn_v2, ch1 - ch2, vdotn_h1, ch1 + ch2,
+    e = earliest_exit(s.zend - z, vz, cv1 - cv2, vdotn_v1, cv1 + cv2, vdotn_v2, ch1 - ch2, vdotn_h1, ch1 + ch2,
                       vdotn_h2);
   const int i = e.i;
   // pending reflectivities: due before this lane reflects again or leaves the guide
@@ -393,7 +394,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
       const bool vert = i <= 2;
       const double wh = vert ? wws : hhh;
       const double vdn = e.d;
-      const double d = 2.0 * vdn / D[16 * s.seg + (vert ? 6 : 8)];
+      const double d = 2.0 * vdn / D[16 * s.seg + (vert ? 8 : 10)];
       const double dl = d * l_seg;
       s.vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
       s.vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;

@@ -233,12 +233,13 @@ void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, .
 // Tapered guide: one 16-double record per segment from the component's
 // 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf):
 // [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf * l_seg
-// (the products of cv1 and ch1, guide_tapering.py:202-204), [6] nlen2 and [7]
-// its sqrt for the vertical mirrors, [8] and [9] for the horizontal ones,
-// [10] h1, [11] whalf.
+// (the products of cv1 and ch1, guide_tapering.py:202-204), [6] zr = seg *
+// l_seg and [7] zr + l_seg (:167, :205), [8] nlen2 and [9] its sqrt for the
+// vertical mirrors, [10] and [11] for the horizontal ones, [12] h1, [13] whalf.
 constexpr int64_t kTaperRec = 16;
 
-void taper_record(double l_seg, double w1, double h1, double ww, double hh, double whalf, double hhalf, double* r) {
+void taper_record(double l_seg, int64_t seg, double w1, double h1, double ww, double hh, double whalf, double hhalf,
+                  double* r) {
   const double l2 = l_seg * l_seg;
   const double w2 = ww * ww, h2 = hh * hh;
   r[0] = w1;
@@ -247,19 +248,21 @@ void taper_record(double l_seg, double w1, double h1, double ww, double hh, doub
   r[3] = hh;
   r[4] = whalf * l_seg;
   r[5] = hhalf * l_seg;
-  r[6] = l2 + w2;
-  r[7] = std::sqrt(r[6]);
-  r[8] = l2 + h2;
+  r[6] = (double)seg * l_seg;
+  r[7] = r[6] + l_seg;
+  r[8] = l2 + w2;
   r[9] = std::sqrt(r[8]);
-  r[10] = h1;
-  r[11] = whalf;
-  for (int k = 12; k < kTaperRec; ++k) r[k] = 0;
+  r[10] = l2 + h2;
+  r[11] = std::sqrt(r[10]);
+  r[12] = h1;
+  r[13] = whalf;
+  for (int k = 14; k < kTaperRec; ++k) r[k] = 0;
 }
 
 void taper_records(double l_seg, int64_t nseg, const double* T, double* D) {  // T: 10 x nseg
   for (int64_t k = 0; k < nseg; ++k)
-    taper_record(l_seg, T[k], T[2 * nseg + k], T[4 * nseg + k], T[5 * nseg + k], T[6 * nseg + k], T[7 * nseg + k],
-                 D + kTaperRec * k);
+    taper_record(l_seg, k, T[k], T[2 * nseg + k], T[4 * nseg + k], T[5 * nseg + k], T[6 * nseg + k],
+                 T[7 * nseg + k], D + kTaperRec * k);
 }
 
 void cross3(const double* a, const double* b, double* o) {
@@ -739,7 +742,7 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
     double* D = h->blob.data() + h->args.doff[comp];
     for (int64_t k = 0; k < nseg; ++k) {
       double* r = D + kTaperRec * k;
-      taper_record(params[0], r[0], r[10], r[2], r[3], r[11], r[1], r);
+      taper_record(params[0], k, r[0], r[12], r[2], r[3], r[13], r[1], r);
     }
     HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp], D, sizeof(double) * kTaperRec * nseg, hipMemcpyHostToDevice));
   }
