@@ -238,7 +238,7 @@ struct QueueRngLDS {
       p[MCV_QCAP];
   int owner[MCV_QCAP];  // stream slot: lane * MCV_SPL + stream of the lane
   uint64_t st0[MCV_BLOCK * MCV_SPL], st1[MCV_BLOCK * MCV_SPL];
-  int64_t next[MCV_BLOCK * MCV_SPL];
+  int64_t next[MCV_BLOCK * MCV_SPL], end[MCV_BLOCK * MCV_SPL];
   int returned[MCV_BLOCK * MCV_SPL];
   int count;  // slot allocator
   int pop;    // next entry a refilled phase B hands out
@@ -358,17 +358,29 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
 // Phase B draws too: a surviving neutron carries its stream's state into the
 // queue and that stream pauses until phase B hands the state back, so every
 // stream still draws for its neutrons one after another in the reference's
-// order.  Each lane owns MCV_SPL streams and produces from any of them that is
+// order.  Each lane owns MCV_SPL stream slots and produces from any that is
 // not paused, so lanes keep producing while their earlier neutrons wait.
 // Phase B runs over a full workgroup of queued neutrons, or over everything
-// queued when no lane can produce.  `tid` is the lane's first stream.
+// queued when no lane can produce.  `tid` is the lane's first stream (slot k
+// starts with stream tid + k * MCV_BLOCK).
+//
+// With a stream pool (`pool` non-null: a launch-wide counter, streams from
+// `pool_base` on), a slot whose stream is exhausted takes the next stream of
+// the pool, so a persistent grid keeps its queue filling until the pool is
+// dry.  Without it, the streams of a workgroup run out together and the last
+// phase-B batches of every workgroup run partly empty (the stream pauses make
+// the queue fill ever slower towards the end).  Which lane runs a stream
+// changes nothing in its neutrons.
 template <class Chain>
-__device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q) {
+__device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q,
+                                                        unsigned long long* pool = nullptr, int64_t pool_base = 0) {
   const int lt = threadIdx.x;
+  const unsigned lane = threadIdx.x & 63u;
   constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
   lds_hist_zero(A);
   if (lt == 0) q.count = 0;
-  unsigned busy = 0, done = 0;  // per stream of this lane: paused / no neutrons left
+  unsigned busy = 0, done = 0;  // per stream slot of this lane: paused / no neutrons left
+  bool dry = pool == nullptr;   // the pool has no stream left for this lane
   int64_t mine = 0;
 #pragma unroll
   for (int k = 0; k < MCV_SPL; ++k) {
@@ -379,6 +391,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     q.st0[o] = r.s0;
     q.st1[o] = r.s1;
     q.next[o] = idx;
+    q.end[o] = end;
     q.returned[o] = 0;
     mine += end - idx;
     if (idx >= end) done |= 1u << k;
@@ -386,6 +399,36 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
   int par = 0, cnt = 0, rr = 0;
   __syncthreads();
   while (true) {
+    // an exhausted, unpaused slot takes the pool's next stream (one atomic per wave)
+    if (!dry) {
+      const unsigned freeslots = done & ~busy & ALL;
+      const bool want = freeslots != 0u;
+      const unsigned long long m = __ballot(want);
+      if (m) {
+        const int first = __builtin_ctzll(m);
+        unsigned long long base = 0;
+        if ((int)lane == first) base = atomicAdd(pool, (unsigned long long)__popcll(m));
+        base = __shfl(base, first, 64);
+        if (want) {
+          const int64_t sid = pool_base + (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+          Rng r;
+          int64_t idx, end;
+          stream_range(A, sid, r, idx, end);
+          if (idx < end) {
+            const int k = __builtin_ctz(freeslots);
+            const int o = lt * MCV_SPL + k;
+            q.st0[o] = r.s0;
+            q.st1[o] = r.s1;
+            q.next[o] = idx;
+            q.end[o] = end;
+            mine += end - idx;
+            done &= ~(1u << k);
+          } else {
+            dry = true;  // streams past the last non-empty one are empty
+          }
+        }
+      }
+    }
     // a stream that is neither paused nor exhausted, round robin from rr
     const unsigned avail = ~(busy | done) & ALL;
     bool has = false;
@@ -408,10 +451,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
       q.st0[o] = w.s0;
       q.st1[o] = w.s1;
       q.next[o] = my + 1;
-      int64_t s_idx, s_end;
-      Rng unused;
-      stream_range(A, tid + (int64_t)sel * MCV_BLOCK, unused, s_idx, s_end);
-      if (my + 1 >= s_end) done |= 1u << sel;
+      if (my + 1 >= q.end[o]) done |= 1u << sel;
       if (has) busy |= 1u << sel;
       rr = (sel + 1) % MCV_SPL;
     }
@@ -422,7 +462,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     }
     int n_new;
     bool can_a;
-    round_totals(q, par, has, (~(busy | done) & ALL) != 0, n_new, can_a);
+    round_totals(q, par, has, (~(busy | done) & ALL) != 0 || (!dry && (done & ~busy & ALL) != 0), n_new, can_a);
     cnt += n_new;  // < 2 * MCV_BLOCK: a full workgroup is taken as soon as one is queued
     if (cnt >= MCV_BLOCK || (!can_a && cnt > 0)) {
       const int take = cnt >= MCV_BLOCK ? MCV_BLOCK : cnt;
@@ -739,10 +779,12 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   }
 
 #define MCV_DEFINE_KERNELS_COMPACTED_RNG(NAME, CHAIN)                                          \
+  extern "C" __device__ int mcv_stream_pool = 1;                                              \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     __shared__ mcv::QueueRngLDS q;                                                            \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x; \
-    mcv::run_block_compacted_rng<CHAIN>(A, tid, q);                                           \
+    mcv::run_block_compacted_rng<CHAIN>(A, tid, q, A.pool,                                    \
+                                        A.tid_base + (int64_t)gridDim.x * (MCV_BLOCK * MCV_SPL)); \
   }                                                                                           \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     __shared__ mcv::QueueRngLDS q;                                                            \

@@ -17,6 +17,7 @@ struct LaunchArgs {
   double* H;
   const uint64_t* rng;               // (nthreads, 2) stream states
   unsigned long long* counter;       // processed neutrons
+  unsigned long long* pool;          // streams taken from the pool (persistent stream-carrying kernels)
   double* dump;                      // (dump_count, 10) final states, or null
   int32_t* dump_alive;               // (dump_count)
   int64_t dump_first, dump_count;

@@ -535,6 +535,7 @@ struct mcv_instr {
   int ncomp = 0;
   int block = 256;  // workgroup size of the code object's kernels
   int spl = 1;      // random streams per lane (a workgroup serves block * spl streams)
+  int pool = 0;     // the kernel takes streams from a pool (persistent grid)
   std::vector<int> types;
   std::vector<int> nparams;  // reference-order params per component (settable)
   mcv::LaunchArgs args{};
@@ -610,6 +611,18 @@ int module_block(hipModule_t mod, int* block, int* spl = nullptr) {
   return MCV_OK;
 }
 
+// Stream-carrying kernels with a stream pool run a persistent grid: as many
+// workgroups as the device holds at once (occupancy x CUs), at most `needed`.
+int64_t persistent_grid(hipFunction_t fn, int block, size_t lds_bytes, int64_t needed) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds_bytes) != hipSuccess ||
+      cus < 1 || per_cu < 1)
+    return needed;
+  return std::min<int64_t>(needed, (int64_t)cus * per_cu);
+}
+
 void release(mcv_instr* h) {
   if (!h) return;
   if (h->d_blob) (void)hipFree(h->d_blob);
@@ -664,14 +677,16 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
     HIPCHECK(hipMalloc(&h->d_rep, h->rep_total * sizeof(double)));
     HIPCHECK(hipMemset(h->d_rep, 0, h->rep_total * sizeof(double)));
   }
-  HIPCHECK(hipMalloc(&h->d_counter, sizeof(unsigned long long)));
+  HIPCHECK(hipMalloc(&h->d_counter, 2 * sizeof(unsigned long long)));  // processed, stream pool
 
   hipError_t e = hipModuleLoad(&h->mod, code_object);
   if (e != hipSuccess) return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
   e = hipModuleGetFunction(&h->fn, h->mod, kernel_name);
   if (e != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s: %s", kernel_name, code_object, hipGetErrorString(e));
-  return module_block(h->mod, &h->block, &h->spl);
+  rc = module_block(h->mod, &h->block, &h->spl);
+  if (rc) return rc;
+  return module_int(h->mod, "mcv_stream_pool", 0, &h->pool);
 }
 
 void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks, int64_t& nthreads, int64_t& npt) {
@@ -768,7 +783,7 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
     HIPCHECK(hipMalloc(&h->d_rng, (size_t)nthreads * 2 * sizeof(uint64_t)));
     h->rng_cap = nthreads;
   }
-  HIPCHECK(hipMemsetAsync(h->d_counter, 0, sizeof(unsigned long long), h->stream));
+  HIPCHECK(hipMemsetAsync(h->d_counter, 0, 2 * sizeof(unsigned long long), h->stream));
   HIPCHECK(hipEventRecord(h->ev[0], h->stream));
   int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
   if (rc) return rc;
@@ -780,6 +795,7 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   A.R = h->d_rep;
   A.rng = h->d_rng;
   A.counter = h->d_counter;
+  A.pool = h->pool ? h->d_counter + 1 : nullptr;
   A.dump = h->d_dump;
   A.dump_alive = h->d_alive;
   A.dump_first = h->dump_first;
@@ -791,7 +807,8 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
   A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
   const unsigned bs = (unsigned)h->block;
   const int64_t per_group = (int64_t)h->block * h->spl;
-  const int64_t grid = (nthreads + per_group - 1) / per_group;
+  int64_t grid = (nthreads + per_group - 1) / per_group;
+  if (h->pool) grid = persistent_grid(h->fn, h->block, A.lds_total * sizeof(double), grid);
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, (unsigned)(A.lds_total * sizeof(double)),
                                  h->stream, kargs, nullptr));
@@ -1106,6 +1123,7 @@ void mcv_instr_destroy(mcv_instr* h) { release(h); }
 // ------------------------------------------------------------------ batched scans
 struct mcv_batch {
   int ninst = 0, ncomp = 0, block = 256, spl = 1;
+  int pool = 0;  // the single-instance kernel takes streams from a pool
   mcv::LaunchArgs args{};
   int64_t pstride = 0, hstride = 0;
   std::vector<int64_t> nbins;
@@ -1114,7 +1132,7 @@ struct mcv_batch {
   uint64_t* d_rng = nullptr;
   int64_t rng_cap = 0;
   int64_t* d_meta = nullptr;                 // item0 (ninst+1), n, nthreads, npt (ninst each)
-  unsigned long long* d_counters = nullptr;  // processed (ninst), work item counter (1)
+  unsigned long long* d_counters = nullptr;  // processed (ninst), work item counter, stream pool
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr, fn_batch = nullptr;
   hipStream_t stream = nullptr;
@@ -1190,7 +1208,7 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   HIPCHECK(hipMalloc(&b->d_hist, hn * sizeof(double)));
   HIPCHECK(hipMemset(b->d_hist, 0, hn * sizeof(double)));
   HIPCHECK(hipMalloc(&b->d_meta, (size_t)(4 * ninst + 1) * sizeof(int64_t)));
-  HIPCHECK(hipMalloc(&b->d_counters, (size_t)(ninst + 1) * sizeof(unsigned long long)));
+  HIPCHECK(hipMalloc(&b->d_counters, (size_t)(ninst + 2) * sizeof(unsigned long long)));
   hipError_t e = hipModuleLoad(&b->mod, code_object);
   if (e != hipSuccess) return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
   if (hipModuleGetFunction(&b->fn, b->mod, kernel_name) != hipSuccess)
@@ -1198,7 +1216,9 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   std::string bn = std::string(kernel_name) + "_batch";
   if (hipModuleGetFunction(&b->fn_batch, b->mod, bn.c_str()) != hipSuccess)
     return fail(MCV_ERR_MODULE, "kernel %s not in %s", bn.c_str(), code_object);
-  return module_block(b->mod, &b->block, &b->spl);
+  int rc = module_block(b->mod, &b->block, &b->spl);
+  if (rc) return rc;
+  return module_int(b->mod, "mcv_stream_pool", 0, &b->pool);
 }
 
 int ensure_rng(mcv_batch* b, int64_t n) {
@@ -1365,11 +1385,13 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     rc = seed_streams(b->stream, seed, nt, b->d_rng);
     if (rc) return rc;
     hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, b->stream, b->d_counters, (int64_t)1);
+    hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, b->stream, b->d_counters + b->ninst + 1, (int64_t)1);
     mcv::LaunchArgs A = b->args;
     A.P = b->d_blob + k * b->pstride;
     A.H = b->d_hist + k * b->hstride;
     A.rng = b->d_rng;
     A.counter = b->d_counters;
+    A.pool = b->pool ? b->d_counters + b->ninst + 1 : nullptr;
     A.dump = nullptr;
     A.dump_alive = nullptr;
     A.dump_count = 0;
@@ -1380,7 +1402,9 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
     const int64_t per_group = (int64_t)b->block * b->spl;
-    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)((nt + per_group - 1) / per_group), 1, 1, (unsigned)b->block, 1,
+    int64_t grid = (nt + per_group - 1) / per_group;
+    if (b->pool) grid = persistent_grid(b->fn, b->block, A.lds_total * sizeof(double), grid);
+    HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)grid, 1, 1, (unsigned)b->block, 1,
                                    1, (unsigned)(A.lds_total * sizeof(double)), b->stream, kargs,
                                    nullptr));
     HIPCHECK(hipEventRecord(b->ev[1], b->stream));
