@@ -173,7 +173,8 @@ struct QueueAdjLDS : QueueRngLDS {
 
 template <class Chain>
 __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const AdjointArgs& J, int64_t tid,
-                                                      QueueAdjLDS<Chain::NX>& q) {
+                                                      QueueAdjLDS<Chain::NX>& q, unsigned long long* pool = nullptr,
+                                                      int64_t pool_base = 0) {
   const int lt = threadIdx.x;
   constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
   double acc[Chain::NS];
@@ -181,23 +182,13 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
   for (int k = 0; k < Chain::NS; ++k) acc[k] = 0;
   if (lt == 0) q.count = 0;
   unsigned busy = 0, done = 0;
+  bool dry = pool == nullptr;
   int64_t mine = 0;
-#pragma unroll
-  for (int k = 0; k < MCV_SPL; ++k) {
-    Rng r;
-    int64_t idx, end;
-    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
-    const int o = lt * MCV_SPL + k;
-    q.st0[o] = r.s0;
-    q.st1[o] = r.s1;
-    q.next[o] = idx;
-    q.returned[o] = 0;
-    mine += end - idx;
-    if (idx >= end) done |= 1u << k;
-  }
+  slots_init(A, q, tid, done, mine);
   int par = 0, cnt = 0, rr = 0;
   __syncthreads();
   while (true) {
+    if (!dry) slots_refill(A, q, pool, pool_base, INT64_MAX, busy, done, dry, mine);
     const unsigned avail = ~(busy | done) & ALL;
     bool has = false;
     Neutron n;
@@ -219,10 +210,7 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
       q.st0[o] = w.s0;
       q.st1[o] = w.s1;
       q.next[o] = my + 1;
-      int64_t s_idx, s_end;
-      Rng unused;
-      stream_range(A, tid + (int64_t)sel * MCV_BLOCK, unused, s_idx, s_end);
-      if (my + 1 >= s_end) done |= 1u << sel;
+      if (my + 1 >= q.end[o]) done |= 1u << sel;
       if (has) busy |= 1u << sel;
       rr = (sel + 1) % MCV_SPL;
     }
@@ -235,7 +223,7 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
     }
     int n_new;
     bool can_a;
-    round_totals(q, par, has, (~(busy | done) & ALL) != 0, n_new, can_a);
+    round_totals(q, par, has, (~(busy | done) & ALL) != 0 || (!dry && (done & ~busy & ALL) != 0), n_new, can_a);
     cnt += n_new;
     if (cnt >= MCV_BLOCK || (!can_a && cnt > 0)) {
       const int take = cnt >= MCV_BLOCK ? MCV_BLOCK : cnt;
@@ -284,10 +272,12 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
 }  // namespace mcv
 
 #define MCV_DEFINE_ADJOINT_KERNEL_COMPACTED(NAME, CHAIN)                                                \
+  extern "C" __device__ int mcv_stream_pool = 1;                                                       \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A, const mcv::AdjointArgs J) { \
     __shared__ mcv::QueueAdjLDS<CHAIN::NX> q;                                                          \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x;      \
-    mcv::run_block_adjoint_rng<CHAIN>(A, J, tid, q);                                                   \
+    mcv::run_block_adjoint_rng<CHAIN>(A, J, tid, q, A.pool,                                            \
+                                      A.tid_base + (int64_t)gridDim.x * (MCV_BLOCK * MCV_SPL));        \
   }
 
 #define MCV_DEFINE_ADJOINT_KERNEL(NAME, CHAIN)                                                          \

@@ -38,6 +38,13 @@
 #define MCV_SPL 1
 #endif
 
+// Streams per work item of the stream-carrying batch kernels: a workgroup
+// refills its stream slots from its item (an LDS counter), so an item of
+// several workgroups' worth of streams ends in one drained queue, not several.
+#ifndef MCV_RNG_ITEM_STREAMS
+#define MCV_RNG_ITEM_STREAMS (8 * MCV_BLOCK * MCV_SPL)
+#endif
+
 extern "C" __device__ int mcv_block_size = MCV_BLOCK;
 extern "C" __device__ int mcv_streams_per_lane = MCV_SPL;
 
@@ -355,6 +362,65 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
   __syncthreads();  // the queue is reused by the next block of streams
 }
 
+// Stream slots of the stream-carrying kernels (QueueRngLDS and the queues
+// derived from it): slot k of lane lt starts with stream tid + k * MCV_BLOCK;
+// `done` marks slots whose stream has no neutron left.
+template <class Q>
+__device__ __forceinline__ void slots_init(const LaunchArgs& A, Q& q, int64_t tid, unsigned& done, int64_t& mine) {
+  const int lt = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < MCV_SPL; ++k) {
+    Rng r;
+    int64_t idx, end;
+    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
+    const int o = lt * MCV_SPL + k;
+    q.st0[o] = r.s0;
+    q.st1[o] = r.s1;
+    q.next[o] = idx;
+    q.end[o] = end;
+    q.returned[o] = 0;
+    mine += end - idx;
+    if (idx >= end) done |= 1u << k;
+  }
+}
+
+// An exhausted, unpaused slot of this lane takes the pool's next stream (one
+// atomic per wave; the pool counter may live in global memory or LDS); a lane
+// that draws an empty stream or one at or past `pool_end` marks the pool dry
+// (streams past the last non-empty one are empty).
+template <class Q>
+__device__ __forceinline__ void slots_refill(const LaunchArgs& A, Q& q, unsigned long long* pool, int64_t pool_base,
+                                             int64_t pool_end, unsigned busy, unsigned& done, bool& dry,
+                                             int64_t& mine) {
+  constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned freeslots = done & ~busy & ALL;
+  const bool want = freeslots != 0u;
+  const unsigned long long m = __ballot(want);
+  if (!m) return;
+  const int first = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if ((int)lane == first) base = atomicAdd(pool, (unsigned long long)__popcll(m));
+  base = __shfl(base, first, 64);
+  if (!want) return;
+  const int64_t sid = pool_base + (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+  Rng r;
+  int64_t idx = 0, end = 0;
+  if (sid < pool_end) stream_range(A, sid, r, idx, end);
+  if (idx < end) {
+    const int k = __builtin_ctz(freeslots);
+    const int o = (int)threadIdx.x * MCV_SPL + k;
+    q.st0[o] = r.s0;
+    q.st1[o] = r.s1;
+    q.next[o] = idx;
+    q.end[o] = end;
+    mine += end - idx;
+    done &= ~(1u << k);
+  } else {
+    dry = true;
+  }
+}
+
 // Phase B draws too: a surviving neutron carries its stream's state into the
 // queue and that stream pauses until phase B hands the state back, so every
 // stream still draws for its neutrons one after another in the reference's
@@ -373,62 +439,20 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
 // changes nothing in its neutrons.
 template <class Chain>
 __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int64_t tid, QueueRngLDS& q,
-                                                        unsigned long long* pool = nullptr, int64_t pool_base = 0) {
+                                                        unsigned long long* pool = nullptr, int64_t pool_base = 0,
+                                                        int64_t pool_end = INT64_MAX) {
   const int lt = threadIdx.x;
-  const unsigned lane = threadIdx.x & 63u;
   constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
   lds_hist_zero(A);
   if (lt == 0) q.count = 0;
   unsigned busy = 0, done = 0;  // per stream slot of this lane: paused / no neutrons left
   bool dry = pool == nullptr;   // the pool has no stream left for this lane
   int64_t mine = 0;
-#pragma unroll
-  for (int k = 0; k < MCV_SPL; ++k) {
-    Rng r;
-    int64_t idx, end;
-    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
-    const int o = lt * MCV_SPL + k;
-    q.st0[o] = r.s0;
-    q.st1[o] = r.s1;
-    q.next[o] = idx;
-    q.end[o] = end;
-    q.returned[o] = 0;
-    mine += end - idx;
-    if (idx >= end) done |= 1u << k;
-  }
+  slots_init(A, q, tid, done, mine);
   int par = 0, cnt = 0, rr = 0;
   __syncthreads();
   while (true) {
-    // an exhausted, unpaused slot takes the pool's next stream (one atomic per wave)
-    if (!dry) {
-      const unsigned freeslots = done & ~busy & ALL;
-      const bool want = freeslots != 0u;
-      const unsigned long long m = __ballot(want);
-      if (m) {
-        const int first = __builtin_ctzll(m);
-        unsigned long long base = 0;
-        if ((int)lane == first) base = atomicAdd(pool, (unsigned long long)__popcll(m));
-        base = __shfl(base, first, 64);
-        if (want) {
-          const int64_t sid = pool_base + (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
-          Rng r;
-          int64_t idx, end;
-          stream_range(A, sid, r, idx, end);
-          if (idx < end) {
-            const int k = __builtin_ctz(freeslots);
-            const int o = lt * MCV_SPL + k;
-            q.st0[o] = r.s0;
-            q.st1[o] = r.s1;
-            q.next[o] = idx;
-            q.end[o] = end;
-            mine += end - idx;
-            done &= ~(1u << k);
-          } else {
-            dry = true;  // streams past the last non-empty one are empty
-          }
-        }
-      }
-    }
+    if (!dry) slots_refill(A, q, pool, pool_base, pool_end, busy, done, dry, mine);
     // a stream that is neither paused nor exhausted, round robin from rr
     const unsigned avail = ~(busy | done) & ALL;
     bool has = false;
@@ -726,7 +750,8 @@ __device__ __forceinline__ void run_block_compacted_rng_refill(const LaunchArgs&
 
 // ------------------------------------------------------------------ batched scans
 // Next (instance, stream block) for the workgroup; false when all are taken.
-__device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int64_t& tid) {
+__device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int64_t& tid,
+                                          int64_t item_streams = MCV_BLOCK * MCV_SPL) {
   __shared__ int64_t s_item;
   if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(B.work, 1ull);
   __syncthreads();
@@ -748,7 +773,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   A.counter = B.inst_processed + lo;
   A.dump = nullptr;
   A.dump_count = 0;
-  tid = (item - B.item0[lo]) * (MCV_BLOCK * MCV_SPL) + threadIdx.x;
+  tid = (item - B.item0[lo]) * item_streams + threadIdx.x;
   return true;
 }
 
@@ -780,6 +805,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 
 #define MCV_DEFINE_KERNELS_COMPACTED_RNG(NAME, CHAIN)                                          \
   extern "C" __device__ int mcv_stream_pool = 1;                                              \
+  extern "C" __device__ int mcv_item_streams = MCV_RNG_ITEM_STREAMS;                          \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
     __shared__ mcv::QueueRngLDS q;                                                            \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x; \
@@ -788,9 +814,15 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   }                                                                                           \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     __shared__ mcv::QueueRngLDS q;                                                            \
+    __shared__ unsigned long long item_pool;                                                  \
     mcv::LaunchArgs A;                                                                        \
     int64_t tid;                                                                              \
-    while (mcv::next_item(B, A, tid)) mcv::run_block_compacted_rng<CHAIN>(A, tid, q);         \
+    while (mcv::next_item(B, A, tid, MCV_RNG_ITEM_STREAMS)) {                                  \
+      const int64_t first = tid - threadIdx.x;                                                \
+      if (threadIdx.x == 0) item_pool = 0;                                                    \
+      mcv::run_block_compacted_rng<CHAIN>(A, tid, q, &item_pool, first + MCV_BLOCK * MCV_SPL, \
+                                          first + MCV_RNG_ITEM_STREAMS);                      \
+    }                                                                                         \
   }
 
 #define MCV_DEFINE_PROCESS_KERNEL(NAME, CHAIN)                                                 \

@@ -562,6 +562,7 @@ struct mcv_instr {
   std::string adj_key;
   int adj_block = 256;
   int adj_spl = 1;
+  int adj_pool = 0;
   double* d_G = nullptr;
   double* d_grad = nullptr;
   int64_t grad_cap = 0;
@@ -895,6 +896,7 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     if (hipModuleGetFunction(&h->adj_fn, h->adj_mod, kernel_name) != hipSuccess)
       return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
     int rc = module_block(h->adj_mod, &h->adj_block, &h->adj_spl);
+    if (!rc) rc = module_int(h->adj_mod, "mcv_stream_pool", 0, &h->adj_pool);
     if (rc) return rc;
     h->adj_key = key;
   }
@@ -931,7 +933,7 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
   if (ndense > 0)
     hipLaunchKernelGGL(mcv_zero_f64, dim3((unsigned)std::min<int64_t>((ndense + 255) / 256, 4096)), dim3(256), 0,
                        h->stream, h->d_dense, ndense);
-  hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, h->stream, h->d_counter, (int64_t)1);
+  hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, h->stream, h->d_counter, (int64_t)2);
   HIPCHECK(hipGetLastError());
   if (ncount > 0) {
     HIPCHECK(hipEventRecord(h->ev[0], h->stream));
@@ -943,6 +945,7 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     A.H = nullptr;
     A.rng = h->d_rng;
     A.counter = h->d_counter;
+    A.pool = h->adj_pool ? h->d_counter + 1 : nullptr;
     A.dump = nullptr;
     A.dump_alive = nullptr;
     A.dump_count = 0;
@@ -956,9 +959,10 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     mcv::AdjointArgs J{h->d_G, h->d_grad, ndense > 0 ? h->d_dense : nullptr};
     const unsigned bs = (unsigned)h->adj_block;
     const int64_t per_group = (int64_t)h->adj_block * h->adj_spl;
+    int64_t grid = (nthreads + per_group - 1) / per_group;
+    if (h->adj_pool) grid = persistent_grid(h->adj_fn, h->adj_block, 0, grid);
     void* kargs[] = {&A, &J};
-    HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)((nthreads + per_group - 1) / per_group), 1, 1, bs, 1, 1, 0,
-                                   h->stream, kargs, nullptr));
+    HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)grid, 1, 1, bs, 1, 1, 0, h->stream, kargs, nullptr));
     HIPCHECK(hipEventRecord(h->ev[2], h->stream));
   }
   HIPCHECK(hipStreamSynchronize(h->stream));
@@ -1123,7 +1127,8 @@ void mcv_instr_destroy(mcv_instr* h) { release(h); }
 // ------------------------------------------------------------------ batched scans
 struct mcv_batch {
   int ninst = 0, ncomp = 0, block = 256, spl = 1;
-  int pool = 0;  // the single-instance kernel takes streams from a pool
+  int pool = 0;          // the single-instance kernel takes streams from a pool
+  int item_streams = 0;  // streams per work item of the batch kernel
   mcv::LaunchArgs args{};
   int64_t pstride = 0, hstride = 0;
   std::vector<int64_t> nbins;
@@ -1218,6 +1223,10 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
     return fail(MCV_ERR_MODULE, "kernel %s not in %s", bn.c_str(), code_object);
   int rc = module_block(b->mod, &b->block, &b->spl);
   if (rc) return rc;
+  rc = module_int(b->mod, "mcv_item_streams", b->block * b->spl, &b->item_streams);
+  if (rc) return rc;
+  if (b->item_streams < b->block * b->spl || b->item_streams % (b->block * b->spl))
+    return fail(MCV_ERR_MODULE, "code object built for %d streams per work item", b->item_streams);
   return module_int(b->mod, "mcv_stream_pool", 0, &b->pool);
 }
 
@@ -1298,7 +1307,7 @@ int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t 
   for (int k = 0; k < n; ++k) {
     int64_t nb, nt, npt;
     partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
-    const int64_t per_item = (int64_t)b->block * b->spl;
+    const int64_t per_item = b->item_streams;
     meta[k + 1] = meta[k] + (nt + per_item - 1) / per_item;
     meta[n + 1 + k] = (int64_t)ncounts[k];
     meta[2 * n + 1 + k] = nt;
