@@ -6,10 +6,11 @@ parameter is a Python loop of separate runs, each a full kernel launch that
 leaves the GPU mostly idle when its neutron count is small.  `BatchedScan`
 takes instruments that share one chain structure (same component types,
 shapes, kernels, table sizes and histogram sizes; any parameter values) and
-runs them all in one persistent kernel: workgroups pull (instance, one
-workgroup's worth of streams) work items from one atomic counter, so small and
-large instances
-share the chip.  Instance k with ncounts[k] neutrons uses exactly the
+runs them all in one persistent kernel: workgroups pull (instance, block of
+streams) work items from one atomic counter, so small and large instances
+share the chip.  A work item is one workgroup's worth of streams, or eight
+for chains whose later stages draw random numbers (their workgroups refill
+stream slots from the item, so the queue is drained once per item).  Instance k with ncounts[k] neutrons uses exactly the
 partition and random streams a separate `run_instrument(instr_k, ncounts[k],
 seed=seed)` would, so each instance's histograms are identical (up to the
 order of atomic additions) to its one-by-one run.  The launch (stream
