@@ -236,6 +236,8 @@ struct QueueCore {
 // deterministic phase B: up to (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
 struct QueueLDS : QueueCore<MCV_QCAP> {};
 
+static_assert(MCV_BLOCK * MCV_SPL <= 65536, "stream slots must fit the 16-bit owner field");
+
 // drawing phase B: the states and next neutron indices of all the
 // workgroup's streams live here.  A queued neutron names its stream slot;
 // that stream is paused while the neutron waits, so its slot holds the
@@ -243,10 +245,12 @@ struct QueueLDS : QueueCore<MCV_QCAP> {};
 struct QueueRngLDS {
   double x[MCV_QCAP], y[MCV_QCAP], z[MCV_QCAP], vx[MCV_QCAP], vy[MCV_QCAP], vz[MCV_QCAP], t[MCV_QCAP],
       p[MCV_QCAP];
-  int owner[MCV_QCAP];  // stream slot: lane * MCV_SPL + stream of the lane
+  // narrow fields keep a 128-lane, 2-slot workgroup's queue under a sixth of the
+  // CU's LDS, so LDS does not cap residency below the register limit
+  uint16_t owner[MCV_QCAP];  // stream slot: lane * MCV_SPL + stream of the lane
   uint64_t st0[MCV_BLOCK * MCV_SPL], st1[MCV_BLOCK * MCV_SPL];
   int64_t next[MCV_BLOCK * MCV_SPL], end[MCV_BLOCK * MCV_SPL];
-  int returned[MCV_BLOCK * MCV_SPL];
+  uint8_t returned[MCV_BLOCK * MCV_SPL];
   int count;  // slot allocator
   int pop;    // next entry a refilled phase B hands out
   int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
