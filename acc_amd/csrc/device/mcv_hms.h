@@ -6,7 +6,11 @@
 // over compile-time bounded root lists (Shape::K).  The breadth-first neutron
 // buffers of the reference (to_be_scattered, to_be_scattered2, scattered and
 // the output list, M x 10 doubles each) are per-lane arrays indexed at run
-// time, so they live in the lane's private scratch.
+// time, so they live in the lane's private scratch.  The scatterer and its
+// first-encounter loop are inlined into the chain kernel: as separate
+// functions their frames and by-reference neutrons went through scratch at
+// every call (c3, 1e8 neutrons: 75 ms with both out of line, 70 with the
+// first-encounter loop inlined, 60 with both).
 #pragma once
 #include "mcv_samples.h"
 
@@ -220,7 +224,7 @@ __device__ __forceinline__ int interactM1(Rng& rng, const double* P, const doubl
 
 // :128-178
 template <class Shape, int KIND, int M>
-__device__ __noinline__ int interactM_path1(Rng& rng, const double* P, const double* D, Neutron* out, int out_len,
+__device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, Neutron* out, int out_len,
                                             Neutron& n) {
   const double min_prob = P[2];
   const int loops = (int)P[20];
@@ -267,7 +271,7 @@ __device__ __noinline__ int interactM_path1(Rng& rng, const double* P, const dou
 
 // :214-273.  P[19] = max_ms_loops, P[20] = max_ms_loops_path1 (packed iparams).
 template <class Shape, int KIND, int M>
-__device__ __noinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
+__device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
                                          const double* __restrict__ D, Neutron* out) {
   if (is_exiting<Shape>(D, n)) {
     out[0] = n;
