@@ -222,40 +222,63 @@ __device__ __forceinline__ int interactM1(Rng& rng, const double* P, const doubl
   return 2;
 }
 
+// The breadth-first lists hold neutrons without their spins (nothing here
+// changes them, so every neutron of the lists carries the incident one's):
+// 64 instead of 80 bytes of lane scratch per entry.  A list entry is copied
+// into registers before it is worked on, and written back only where the
+// reference stores it into a list.
+struct NeutronS {
+  double x, y, z, vx, vy, vz, t, p;
+};
+
+__device__ __forceinline__ void store_s(NeutronS& d, const Neutron& n) {
+  d.x = n.x; d.y = n.y; d.z = n.z; d.vx = n.vx; d.vy = n.vy; d.vz = n.vz; d.t = n.t; d.p = n.p;
+}
+
+__device__ __forceinline__ Neutron load_s(const NeutronS& d, double s1, double s2) {
+  Neutron n;
+  n.x = d.x; n.y = d.y; n.z = d.z; n.vx = d.vx; n.vy = d.vy; n.vz = d.vz; n.t = d.t; n.p = d.p;
+  n.s1 = s1;
+  n.s2 = s2;
+  return n;
+}
+
 // :128-178
 template <class Shape, int KIND, int M>
-__device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, Neutron* out, int out_len,
-                                            Neutron& n) {
+__device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, NeutronS* out,
+                                               int out_len, const Neutron& n) {
   const double min_prob = P[2];
   const int loops = (int)P[20];
   if (n.p < min_prob) return 0;
-  Neutron bufA[M], bufB[M], scattered[2];
-  Neutron* tbs = bufA;
-  Neutron* tbs2 = bufB;
+  NeutronS bufA[M], bufB[M];
+  Neutron scattered[2];
+  NeutronS* tbs = bufA;
+  NeutronS* tbs2 = bufB;
   int N_tbs = 1;
-  tbs[0] = n;
+  store_s(tbs[0], n);
   int out_index = 0;
   const int lim = M < out_len ? M : out_len;
   for (int iloop = 0; iloop < loops; ++iloop) {
     int s2 = 0;
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
-      const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, tbs[ineutron]);
+      Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
+      const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, cur);
       for (int is = 0; is < ns1; ++is) {
         const Neutron& sc1 = scattered[is];
         if (sc1.p < min_prob) continue;
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) != INSIDE) {
-          out[out_index] = sc1;
+          store_s(out[out_index], sc1);
           out_index += 1;
           if (out_index >= lim) return out_index;
           continue;
         }
-        tbs2[s2] = sc1;
+        store_s(tbs2[s2], sc1);
         s2 += 1;
         if (s2 >= M) break;
       }
       if (s2 >= M) break;
     }
-    Neutron* tmp = tbs;
+    NeutronS* tmp = tbs;
     tbs = tbs2;
     tbs2 = tmp;
     N_tbs = s2;
@@ -272,25 +295,26 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
 // :214-273.  P[19] = max_ms_loops, P[20] = max_ms_loops_path1 (packed iparams).
 template <class Shape, int KIND, int M>
 __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
-                                         const double* __restrict__ D, Neutron* out) {
+                                            const double* __restrict__ D, Neutron* out) {
   if (is_exiting<Shape>(D, n)) {
     out[0] = n;
     return 1;
   }
   const double min_prob = P[2];
   const int loops = (int)P[19];
-  Neutron bufA[M], bufB[M], scattered[M];
-  Neutron* tbs = bufA;
-  Neutron* tbs2 = bufB;
+  NeutronS bufA[M], bufB[M], scattered[M];
+  NeutronS* tbs = bufA;
+  NeutronS* tbs2 = bufB;
   int N_tbs = 1;
-  tbs[0] = n;
+  store_s(tbs[0], n);
   int out_index = 0;
   for (int iloop = 0; iloop < loops; ++iloop) {
     int s2 = 0;
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
-      const int ns1 = interactM_path1<Shape, KIND, M>(rng, P, D, scattered, M, tbs[ineutron]);
+      const Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
+      const int ns1 = interactM_path1<Shape, KIND, M>(rng, P, D, scattered, M, cur);
       for (int is = 0; is < ns1; ++is) {
-        Neutron& sc1 = scattered[is];
+        Neutron sc1 = load_s(scattered[is], n.s1, n.s2);
         if (sc1.p < min_prob) continue;
         if (!is_moving(sc1)) continue;
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) == INSIDE)
@@ -301,13 +325,13 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
           if (out_index >= M) return out_index;
           continue;
         }
-        tbs2[s2] = sc1;
+        store_s(tbs2[s2], sc1);
         s2 += 1;
         if (s2 >= M) break;
       }
       if (s2 >= M) break;
     }
-    Neutron* tmp = tbs;
+    NeutronS* tmp = tbs;
     tbs = tbs2;
     tbs2 = tmp;
     N_tbs = s2;
@@ -315,7 +339,7 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
   }
   for (int i = 0; i < N_tbs; ++i) {
     if (out_index >= M) break;
-    Neutron& nn = tbs[i];
+    Neutron nn = load_s(tbs[i], n.s1, n.s2);
     const double x = nn.x, y = nn.y, z = nn.z;
     propagate_out<Shape>(D, nn);
     const double dist = sqrt((nn.x - x) * (nn.x - x) + (nn.y - y) * (nn.y - y) + (nn.z - z) * (nn.z - z));
