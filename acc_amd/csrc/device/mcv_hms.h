@@ -263,7 +263,10 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
       Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
       const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, cur);
-      for (int is = 0; is < ns1; ++is) {
+      // at most two outputs: constant indices keep them in registers
+#pragma unroll
+      for (int is = 0; is < 2; ++is) {
+        if (is >= ns1) break;
         const Neutron& sc1 = scattered[is];
         if (sc1.p < min_prob) continue;
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) != INSIDE) {
