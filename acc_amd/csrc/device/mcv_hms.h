@@ -235,6 +235,8 @@ __device__ __forceinline__ void store_s(NeutronS& d, const Neutron& n) {
   d.x = n.x; d.y = n.y; d.z = n.z; d.vx = n.vx; d.vy = n.vy; d.vz = n.vz; d.t = n.t; d.p = n.p;
 }
 
+__device__ __forceinline__ void store_s(Neutron& d, const Neutron& n) { d = n; }
+
 __device__ __forceinline__ Neutron load_s(const NeutronS& d, double s1, double s2) {
   Neutron n;
   n.x = d.x; n.y = d.y; n.z = d.z; n.vx = d.vx; n.vy = d.vy; n.vz = d.vz; n.t = d.t; n.p = d.p;
@@ -296,11 +298,13 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
 }
 
 // :214-273.  P[19] = max_ms_loops, P[20] = max_ms_loops_path1 (packed iparams).
-template <class Shape, int KIND, int M>
+// `out` holds Neutron rows (buffered mode) or spin-less NeutronS entries (the
+// fused chains, which restore the incident neutron's spins).
+template <class Shape, int KIND, int M, class Out>
 __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
-                                            const double* __restrict__ D, Neutron* out) {
+                                            const double* __restrict__ D, Out* out) {
   if (is_exiting<Shape>(D, n)) {
-    out[0] = n;
+    store_s(out[0], n);
     return 1;
   }
   const double min_prob = P[2];
@@ -323,7 +327,7 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) == INSIDE)
           propagate_to_next_exiting_surface_with_attenuation<Shape>(P, D, sc1);
         if (is_exiting<Shape>(D, sc1)) {
-          out[out_index] = sc1;
+          store_s(out[out_index], sc1);
           out_index += 1;
           if (out_index >= M) return out_index;
           continue;
@@ -348,7 +352,7 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
     const double dist = sqrt((nn.x - x) * (nn.x - x) + (nn.y - y) * (nn.y - y) + (nn.z - z) * (nn.z - z));
     const double atten = atten_coeff(P, nn);
     nn.p *= exp(-atten * dist);
-    out[out_index] = nn;
+    store_s(out[out_index], nn);
     out_index += 1;
   }
   return out_index;

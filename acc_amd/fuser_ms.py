@@ -53,10 +53,11 @@ def _generate_compacted(comps, m, pre, structure):
   }}
   return -1;""")
     M = comps[m].max_scattered_neutrons
-    b = [f"mcv::Neutron out[{M}];",
+    b = [f"mcv::NeutronS out[{M}];  // outputs without spins: they keep the incident neutron's",
+         "const double s1 = n.s1, s2 = n.s2;",
          "const int num = " + comps[m].propagate.render(comps[m], m) + ";",
          f"""for (int k = 0; k < num; ++k) {{
-    mcv::Neutron o = out[k];
+    mcv::Neutron o = mcv::load_s(out[k], s1, s2);
     do {{
       if (A.gate && !(o.p > 0)) break;
       {_tail(comps, m)}
@@ -115,7 +116,8 @@ def generate_ms_source(instrument, compact=True):
     if m > 0:
         lines.append(f"mcv::apply_transformation(n, P + A.xoff[{m - 1}]);")
     M = comps[m].max_scattered_neutrons
-    lines.append(f"mcv::Neutron out[{M}];")
+    lines.append(f"mcv::NeutronS out[{M}];  // outputs without spins: they keep the incident neutron's")
+    lines.append("const double s1 = n.s1, s2 = n.s2;")
     lines.append("const int num = " + comps[m].propagate.render(comps[m], m) + ";")
     tail = []
     for i in range(m + 1, len(comps)):
@@ -125,7 +127,7 @@ def generate_ms_source(instrument, compact=True):
         tail.append("if (A.gate && !(o.p > 0)) break;")
     tail_text = "\n      ".join(tail)
     lines.append(f"""for (int k = 0; k < num; ++k) {{
-    mcv::Neutron o = out[k];
+    mcv::Neutron o = mcv::load_s(out[k], s1, s2);
     do {{
       if (A.gate && !(o.p > 0)) break;
       {tail_text}
