@@ -3,14 +3,18 @@
 //
 // acc/geometry/propagation.py:9-191 and
 // acc/components/samples/homogeneous_multiple_scatterer.py:63-303, restated
-// over compile-time bounded root lists (Shape::K).  The breadth-first neutron
-// buffers of the reference (to_be_scattered, to_be_scattered2, scattered and
-// the output list, M x 10 doubles each) are per-lane arrays indexed at run
-// time, so they live in the lane's private scratch.  The scatterer and its
-// first-encounter loop are inlined into the chain kernel: as separate
-// functions their frames and by-reference neutrons went through scratch at
-// every call (c3, 1e8 neutrons: 75 ms with both out of line, 70 with the
-// first-encounter loop inlined, 60 with both).
+// over compile-time bounded root lists (Shape::K).  The reference's lists of
+// neutrons still to scatter (to_be_scattered, to_be_scattered2) are per-lane
+// arrays indexed at run time, so they live in the lane's private scratch; its
+// lists of an encounter's outputs (scattered) and of the scatterer's outputs
+// are not kept: each output is handed on as it is produced (hms_scatter).
+// The scatterer and its first-encounter loop are inlined into the chain
+// kernel: as separate functions their frames and by-reference neutrons went
+// through scratch at every call (c3, 1e8 neutrons: 75 ms with both out of
+// line, 70 with the first-encounter loop inlined, 60 with both; 48.2 -> 43.4
+// ms later with the output lists gone).  Keeping each list's first entry in
+// registers measured slower (52 ms: the register file is full, and spills
+// replace the scratch traffic).
 #pragma once
 #include "mcv_samples.h"
 
@@ -245,10 +249,12 @@ __device__ __forceinline__ Neutron load_s(const NeutronS& d, double s1, double s
   return n;
 }
 
-// :128-178
-template <class Shape, int KIND, int M>
-__device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, NeutronS* out,
-                                               int out_len, const Neutron& n) {
+// :128-178.  Each output is handed to emit(const Neutron&) in the order in
+// which the reference stores it into its output list (no list is kept); the
+// return value is the reference's output count.
+template <class Shape, int KIND, int M, class Emit>
+__device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, int out_len,
+                                               const Neutron& n, Emit&& emit) {
   const double min_prob = P[2];
   const int loops = (int)P[20];
   if (n.p < min_prob) return 0;
@@ -272,7 +278,7 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
         const Neutron& sc1 = scattered[is];
         if (sc1.p < min_prob) continue;
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) != INSIDE) {
-          store_s(out[out_index], sc1);
+          emit(sc1);
           out_index += 1;
           if (out_index >= lim) return out_index;
           continue;
@@ -291,25 +297,31 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
   }
   for (int i = 0; i < N_tbs; ++i) {
     if (out_index >= lim) break;
-    out[out_index] = tbs[i];
+    emit(load_s(tbs[i], n.s1, n.s2));
     out_index += 1;
   }
   return out_index;
 }
 
 // :214-273.  P[19] = max_ms_loops, P[20] = max_ms_loops_path1 (packed iparams).
-// `out` holds Neutron rows (buffered mode) or spin-less NeutronS entries (the
-// fused chains, which restore the incident neutron's spins).
-template <class Shape, int KIND, int M, class Out>
-__device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* __restrict__ P,
-                                            const double* __restrict__ D, Out* out) {
+// Outputs go to emit(const Neutron&) in the reference's output-list order.
+// The reference lets the first encounter (interactM_path1) finish, with all
+// its draws, before it reads that encounter's outputs, and none of that
+// reading draws; so reading each output as the encounter hands it over, and
+// ignoring the rest once the reference would stop reading (its list of
+// neutrons to scatter is full, or M outputs are out), keeps every draw and
+// every output.  The incident neutron is taken by value: an emit that
+// overwrites the caller's neutron does not change the spins read here.
+template <class Shape, int KIND, int M, class Emit>
+__device__ __forceinline__ int hms_scatter(Rng& rng, const Neutron n, const double* __restrict__ P,
+                                           const double* __restrict__ D, Emit&& emit) {
   if (is_exiting<Shape>(D, n)) {
-    store_s(out[0], n);
+    emit(n);
     return 1;
   }
   const double min_prob = P[2];
   const int loops = (int)P[19];
-  NeutronS bufA[M], bufB[M], scattered[M];
+  NeutronS bufA[M], bufB[M];
   NeutronS* tbs = bufA;
   NeutronS* tbs2 = bufB;
   int N_tbs = 1;
@@ -317,26 +329,28 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
   int out_index = 0;
   for (int iloop = 0; iloop < loops; ++iloop) {
     int s2 = 0;
+    bool full = false, done = false;  // the reference's break (s2 reached M) and return (M outputs)
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
       const Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
-      const int ns1 = interactM_path1<Shape, KIND, M>(rng, P, D, scattered, M, cur);
-      for (int is = 0; is < ns1; ++is) {
-        Neutron sc1 = load_s(scattered[is], n.s1, n.s2);
-        if (sc1.p < min_prob) continue;
-        if (!is_moving(sc1)) continue;
+      interactM_path1<Shape, KIND, M>(rng, P, D, M, cur, [&](const Neutron& o) {
+        if (full || done) return;
+        Neutron sc1 = o;
+        if (sc1.p < min_prob) return;
+        if (!is_moving(sc1)) return;
         if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) == INSIDE)
           propagate_to_next_exiting_surface_with_attenuation<Shape>(P, D, sc1);
         if (is_exiting<Shape>(D, sc1)) {
-          store_s(out[out_index], sc1);
+          emit(sc1);
           out_index += 1;
-          if (out_index >= M) return out_index;
-          continue;
+          done = out_index >= M;
+          return;
         }
         store_s(tbs2[s2], sc1);
         s2 += 1;
-        if (s2 >= M) break;
-      }
-      if (s2 >= M) break;
+        full = s2 >= M;
+      });
+      if (done) return out_index;
+      if (full) break;
     }
     NeutronS* tmp = tbs;
     tbs = tbs2;
@@ -352,10 +366,23 @@ __device__ __forceinline__ int hms_scatterM(Rng& rng, Neutron& n, const double* 
     const double dist = sqrt((nn.x - x) * (nn.x - x) + (nn.y - y) * (nn.y - y) + (nn.z - z) * (nn.z - z));
     const double atten = atten_coeff(P, nn);
     nn.p *= exp(-atten * dist);
-    store_s(out[out_index], nn);
+    emit(nn);
     out_index += 1;
   }
   return out_index;
+}
+
+// The scatterer with an output list: `out` holds Neutron rows (buffered mode)
+// or spin-less NeutronS entries (fused chains whose downstream components
+// draw, which restore the incident neutron's spins).
+template <class Shape, int KIND, int M, class Out>
+__device__ __forceinline__ int hms_scatterM(Rng& rng, const Neutron& n, const double* __restrict__ P,
+                                            const double* __restrict__ D, Out* out) {
+  int k = 0;
+  return hms_scatter<Shape, KIND, M>(rng, n, P, D, [&](const Neutron& o) {
+    store_s(out[k], o);
+    ++k;
+  });
 }
 
 }  // namespace mcv

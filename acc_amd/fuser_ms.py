@@ -5,7 +5,10 @@ out_neutrons buffer and then runs every downstream component, with its frame
 change, on each output neutron (acc/run_script.py:204-230, template :245-260).
 The same structure is emitted here; the dump records output 0's final state and
 the number of outputs.  As in the reference, one multiple-scattering component
-per chain is supported (run_script.py:211).
+per chain is supported (run_script.py:211).  When no downstream component
+draws random numbers, the downstream components run on each output as the
+scatterer produces it, and no output list is kept (every draw and every
+output is the reference's; only the order of histogram additions changes).
 
 Compacted form (the default): in a wavefront most neutrons miss the sample
 (a hollow cylinder 3 cm across in a 10 cm beam), and the scatterer's bounded
@@ -31,6 +34,44 @@ def _tail(comps, m, n="o"):
     return "\n      ".join(tail)
 
 
+def _scatter_and_tail(comps, m):
+    """Statements running the scatterer and, on each output, the downstream
+    components; the first output's final state is left in `n`.  When nothing
+    downstream draws, each output goes down the chain as the scatterer hands
+    it over (no output list: the draw order per stream is the reference's
+    either way); otherwise the outputs are kept in a list and run afterwards,
+    as the reference does."""
+    c = comps[m]
+    M = c.max_scattered_neutrons
+    tail = _tail(comps, m)
+    if not any(d.propagate.stochastic for d in comps[m + 1:]):
+        call = c.propagate_emit.format(i=m, n="n", emit="emit", **c._render_context(m))
+        return [f"""int k = 0;
+  auto emit = [&](const mcv::Neutron& out) {{
+    mcv::Neutron o = out;
+    do {{
+      if (A.gate && !(o.p > 0)) break;
+      {tail}
+    }} while (0);
+    if (k == 0) n = o;
+    ++k;
+  }};""",
+                f"const int num = {call};",
+                "return num;"]
+    return [f"mcv::NeutronS out[{M}];  // outputs without spins: they keep the incident neutron's",
+            "const double s1 = n.s1, s2 = n.s2;",
+            "const int num = " + c.propagate.render(c, m) + ";",
+            f"""for (int k = 0; k < num; ++k) {{
+    mcv::Neutron o = mcv::load_s(out[k], s1, s2);
+    do {{
+      if (A.gate && !(o.p > 0)) break;
+      {tail}
+    }} while (0);
+    if (k == 0) n = o;
+  }}""",
+            "return num;"]
+
+
 def _generate_compacted(comps, m, pre, structure):
     a = []
     for i in range(m):
@@ -52,19 +93,7 @@ def _generate_compacted(comps, m, pre, structure):
     n = o;
   }}
   return -1;""")
-    M = comps[m].max_scattered_neutrons
-    b = [f"mcv::NeutronS out[{M}];  // outputs without spins: they keep the incident neutron's",
-         "const double s1 = n.s1, s2 = n.s2;",
-         "const int num = " + comps[m].propagate.render(comps[m], m) + ";",
-         f"""for (int k = 0; k < num; ++k) {{
-    mcv::Neutron o = mcv::load_s(out[k], s1, s2);
-    do {{
-      if (A.gate && !(o.p > 0)) break;
-      {_tail(comps, m)}
-    }} while (0);
-    if (k == 0) n = o;
-  }}""",
-         "return num;"]
+    b = _scatter_and_tail(comps, m)
     a_text = "\n  ".join(a)
     b_text = "\n  ".join(b)
     src = f"""// fused instrument kernel (multiple scattering), compacted at the sample: {structure}
@@ -115,26 +144,7 @@ def generate_ms_source(instrument, compact=True):
         lines.append("if (A.gate && !(n.p > 0)) return 0;")
     if m > 0:
         lines.append(f"mcv::apply_transformation(n, P + A.xoff[{m - 1}]);")
-    M = comps[m].max_scattered_neutrons
-    lines.append(f"mcv::NeutronS out[{M}];  // outputs without spins: they keep the incident neutron's")
-    lines.append("const double s1 = n.s1, s2 = n.s2;")
-    lines.append("const int num = " + comps[m].propagate.render(comps[m], m) + ";")
-    tail = []
-    for i in range(m + 1, len(comps)):
-        c = comps[i]
-        tail.append(f"mcv::apply_transformation(o, P + A.xoff[{i - 1}]);")
-        tail.append(c.propagate.render(c, i, n="o"))
-        tail.append("if (A.gate && !(o.p > 0)) break;")
-    tail_text = "\n      ".join(tail)
-    lines.append(f"""for (int k = 0; k < num; ++k) {{
-    mcv::Neutron o = mcv::load_s(out[k], s1, s2);
-    do {{
-      if (A.gate && !(o.p > 0)) break;
-      {tail_text}
-    }} while (0);
-    if (k == 0) n = o;
-  }}""")
-    lines.append("return num;")
+    lines += _scatter_and_tail(comps, m)
     body = "\n  ".join(lines)
     src = f"""// fused instrument kernel (multiple scattering): {structure}
 #include "mcv_kernel.h"
