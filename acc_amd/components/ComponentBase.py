@@ -24,7 +24,7 @@ class DeviceFunction:
     """
 
     def __init__(self, call, stochastic=False, monitor=False, preamble=None, adjoint=None, dparams=None,
-                 binning=None, stepper=None):
+                 binning=None, stepper=None, split=None):
         self.call = call
         self.stochastic = stochastic
         self.monitor = monitor
@@ -40,6 +40,11 @@ class DeviceFunction:
         # `state` (C++ type) and `init` / `step` / `finish` format strings
         # ({st} = the state, {n} = the neutron, `tr` = a tracker in scope)
         self.stepper = stepper
+        # a component whose first part kills most neutrons (a guide entrance):
+        # (`enter`, a bool expression that is false when the neutron dies,
+        # `rest`, the statement(s) completing `call`); a compacted kernel may end
+        # its per-stream phase after `enter`
+        self.split = split
 
     def render(self, comp, i, n="n"):
         return self.call.format(i=i, n=n, **comp._render_context(i))
@@ -49,6 +54,9 @@ class DeviceFunction:
 
     def render_step(self, comp, i, part, st="st", n="n"):
         return self.stepper[part].format(i=i, n=n, st=st, **comp._render_context(i))
+
+    def render_split(self, comp, i, part, n="n"):
+        return self.split[part].format(i=i, n=n, **comp._render_context(i))
 
     def render_binning(self, comp, i, n="n"):
         return self.binning.format(i=i, n=n, **comp._render_context(i))

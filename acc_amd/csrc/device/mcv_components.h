@@ -148,19 +148,31 @@ __device__ __forceinline__ ExitChoice earliest_exit(double n0, double d0, double
 // then the mirror normals' squared lengths and lengths the host derives from
 // them (l*l + ww*ww, its sqrt, l*l + hh*hh, its sqrt: the reference's nlen2
 // and sqrt(nlen2), correctly rounded on either side).
+// The guide runs in two parts: the entrance (move to z = 0, test the opening)
+// and the bounces.  A compacted kernel can end its per-stream phase at the
+// entrance, where three in four neutrons of a beam wider than the opening
+// die, and run the bounces with whole wavefronts of neutrons that entered.
+// guide_enter returns false when it absorbs (p = -1, the rest of the state
+// left as it was); otherwise the neutron is at z = 0.
+__device__ __forceinline__ bool guide_enter(Neutron& n, const double* __restrict__ P) {
+  const double hw1 = P[2], hh1 = P[3];
+  const double dt = -n.z / n.vz;
+  const double x = n.x + n.vx * dt, y = n.y + n.vy * dt;
+  if (x <= -hw1 || x >= hw1 || y <= -hh1 || y >= hh1) {
+    n.p = -1;  // absorb()
+    return false;
+  }
+  n.x = x;
+  n.y = y;
+  n.z = 0.;
+  n.t += dt;
+  return true;
+}
+
 template <class T>
-__device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P, T tr) {
+__device__ __forceinline__ void guide_bounces_t(Neutron& n, const double* __restrict__ P, T tr) {
   const double ww = P[0], hh = P[1], hw1 = P[2], hh1 = P[3], l = P[4];
   double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz, t = n.t, prob = n.p;
-  const double dt = -z / vz;
-  x += vx * dt;
-  y += vy * dt;
-  z = 0.;
-  t += dt;
-  if (x <= -hw1 || x >= hw1 || y <= -hh1 || y >= hh1) {
-    n.p = -1;  // absorb(): the rest of the state is left as it was
-    return;
-  }
   for (int nb = 0; nb < 100000; ++nb) {
     const double av = l * vx, bv = ww * vz, ah = l * vy, bh = hh * vz;
     const double vdotn_v1 = bv + av, vdotn_v2 = bv - av, vdotn_h1 = bh + ah, vdotn_h2 = bh - ah;
@@ -195,7 +207,15 @@ __device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P
   n.p = prob;
 }
 
+template <class T>
+__device__ __forceinline__ void guide_t(Neutron& n, const double* __restrict__ P, T tr) {
+  if (guide_enter(n, P)) guide_bounces_t(n, P, tr);
+}
+
 __device__ __forceinline__ void guide(Neutron& n, const double* __restrict__ P) { guide_t(n, P, NoTrack{}); }
+__device__ __forceinline__ void guide_bounces(Neutron& n, const double* __restrict__ P) {
+  guide_bounces_t(n, P, NoTrack{});
+}
 
 // ------------------------------------------------------------------ Slit / Beamstop
 // acc/components/optics/slit.py:72-105, beamstop.py:70-103.

@@ -86,7 +86,7 @@ def phase_b_draws(components, k):
     return any(c.propagate.stochastic for c in components[k + 1:])
 
 
-def _phase(components, lo, hi, first_transform):
+def _phase_lines(components, lo, hi, first_transform):
     lines = []
     for i in range(lo, hi):
         comp = components[i]
@@ -94,8 +94,11 @@ def _phase(components, lo, hi, first_transform):
             lines.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
         lines.append(comp.propagate.render(comp, i))
         lines.append("if (A.gate && !(n.p > 0)) return 0;")
-    lines.append("return 1;")
-    return "\n  ".join(lines)
+    return lines
+
+
+def _phase(components, lo, hi, first_transform):
+    return "\n  ".join(_phase_lines(components, lo, hi, first_transform) + ["return 1;"])
 
 
 def generate_source(instrument, compact=True, block=None, waves_per_eu=None, streams_per_lane=None, refill=None):
@@ -194,6 +197,31 @@ struct Chain {{
     return src, structure + tag
 
 
+def _split_phases(comps, k):
+    """Phases A and B when component k can stop after its entrance: with
+    gating (dead neutrons dropped), phase A ends with the entrance and phase B
+    starts with the rest of component k, so that its loop runs with whole
+    wavefronts of neutrons that entered; without gating (every neutron runs
+    through every component, absorbed or not) phase A runs all of component k
+    as before, since a neutron absorbed at the entrance must then not run the
+    rest."""
+    c = comps[k]
+    a = _phase_lines(comps, 0, k, False)
+    if k > 0:
+        a.append(f"mcv::apply_transformation(n, P + A.xoff[{k - 1}]);")
+    a.append(f"""if (A.gate) {{
+    if (!{c.propagate.render_split(c, k, 0)}) return 0;
+  }} else {{
+    {c.propagate.render(c, k)}
+  }}""")
+    a.append("if (A.gate && !(n.p > 0)) return 0;")
+    a.append("return 1;")
+    b = [f"if (A.gate) {{ {c.propagate.render_split(c, k, 1)} }}",
+         "if (A.gate && !(n.p > 0)) return 0;",
+         _phase(comps, k + 1, len(comps), True)]
+    return "\n  ".join(a), "\n  ".join(b)
+
+
 def _generate(instrument, compact, refill=True):
     comps = instrument.components
     if not comps:
@@ -212,6 +240,8 @@ def _generate(instrument, compact, refill=True):
     if k is not None:
         a = _phase(comps, 0, k + 1, False)
         b = _phase(comps, k + 1, len(comps), True)
+        if comps[k].propagate.split is not None:
+            a, b = _split_phases(comps, k)
         draws = phase_b_draws(comps, k)
         b_sig = "mcv::Rng& rng, mcv::Neutron& n" if draws else "mcv::Neutron& n"
         b_void = "(void)rng; (void)H;" if draws else "(void)H;"
