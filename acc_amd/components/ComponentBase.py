@@ -42,8 +42,9 @@ class DeviceFunction:
         self.stepper = stepper
         # a component whose first part kills most neutrons (a guide entrance):
         # (`enter`, a bool expression that is false when the neutron dies,
-        # `rest`, the statement(s) completing `call`); a compacted kernel may end
-        # its per-stream phase after `enter`
+        # `rest`, the statement(s) completing `call`, and `rest` in score-tracking
+        # form, as `adjoint` is of `call`); a compacted kernel may end its
+        # per-stream phase after `enter`
         self.split = split
 
     def render(self, comp, i, n="n"):
@@ -55,8 +56,8 @@ class DeviceFunction:
     def render_step(self, comp, i, part, st="st", n="n"):
         return self.stepper[part].format(i=i, n=n, st=st, **comp._render_context(i))
 
-    def render_split(self, comp, i, part, n="n"):
-        return self.split[part].format(i=i, n=n, **comp._render_context(i))
+    def render_split(self, comp, i, part, n="n", s=None):
+        return self.split[part].format(i=i, n=n, s=s, **comp._render_context(i))
 
     def render_binning(self, comp, i, n="n"):
         return self.binning.format(i=i, n=n, **comp._render_context(i))

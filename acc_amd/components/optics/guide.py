@@ -48,7 +48,8 @@ class Guide(ComponentBase):
         "mcv::guide({n}, P + A.off[{i}]);",
         adjoint="mcv::guide_t({n}, P + A.off[{i}], mcv::GuideScore({s}));",
         dparams=lambda comp: [("R0", 5), ("Qc", 6), ("alpha", 7), ("m", 8), ("W", 9)],
-        split=("mcv::guide_enter({n}, P + A.off[{i}])", "mcv::guide_bounces({n}, P + A.off[{i}]);"))
+        split=("mcv::guide_enter({n}, P + A.off[{i}])", "mcv::guide_bounces({n}, P + A.off[{i}]);",
+               "mcv::guide_bounces_t({n}, P + A.off[{i}], mcv::GuideScore({s}));"))
 
     def _mcv_desc(self):
         return dict(type=self.mcv_type, params=np.array(self.propagate_params, dtype=float))

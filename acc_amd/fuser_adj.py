@@ -99,9 +99,28 @@ def generate_adjoint_source(instrument, block=None, compact=True, streams_per_la
         # scores of phase-B components do not exist yet during phase A
         arrays_pa = [a for a in arrays_a if int(a[0][1:]) <= k]
         la = a_views + a_decl + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz; (void)nidx;"]
-        la += _adjoint_lines(comps, 0, k + 1, slot, arrays_pa, False, "0")
-        la.append("return 1;")
         lb = b_views + b_decl + ["mcv::DenseTrack dz;", "dz.set = false;", "(void)rng; (void)dz;"]
+        c = comps[k]
+        if c.propagate.split is not None:
+            # phase A ends at component k's entrance when gating (as the
+            # forward kernel does); its scores then start in phase B
+            la += _adjoint_lines(comps, 0, k, slot, arrays_pa, False, "0")
+            if k > 0:
+                la.append(f"mcv::apply_transformation(n, P + A.xoff[{k - 1}]);")
+            full = (c.propagate.render_adjoint(c, k, "s%d" % k) if k in slot else c.propagate.render(c, k))
+            rest = (c.propagate.render_split(c, k, 2, s="s%d" % k) if k in slot
+                    else c.propagate.render_split(c, k, 1))
+            la.append(f"""if (A.gate) {{
+    if (!{c.propagate.render_split(c, k, 0)}) return 0;
+  }} else {{
+    {full}
+  }}""")
+            la.append("if (A.gate && !(n.p > 0)) return 0;")
+            lb.append(f"if (A.gate) {{ {rest} }}")
+            lb.append("if (A.gate && !(n.p > 0)) return;")
+        else:
+            la += _adjoint_lines(comps, 0, k + 1, slot, arrays_pa, False, "0")
+        la.append("return 1;")
         lb += _adjoint_lines(comps, k + 1, len(comps), slot, arrays_a, True, "")
         src = f"""// adjoint of the fused instrument kernel, compacted after component {k}:
 // {" -> ".join(type(c).__name__ for c in comps)}
