@@ -84,7 +84,7 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
 // chain of strict "t2 < t1" tests over the walls the neutron moves towards
 // (d_k < 0; guide.py:100-123, guide_tapering.py:206-230).  Here the chain
 // compares quotients by cross-multiplication, without branches: a losing
-// candidate costs three products and two selects instead of an FP64 division
+// candidate costs two products and a few selects instead of an FP64 division
 // (the costliest instruction sequence of the loop, which a wavefront would
 // execute whenever any of its lanes takes the branch).  The winning wall's
 // time is divided afterwards from the same operands as in the reference, so
@@ -97,14 +97,11 @@ struct ExitChoice {
   bool odd;
 };
 
+// e.d < 0 (see earliest_exit) and d < 0, so d * e.d > 0 and n/d < e.n/e.d
+// is n * e.d < e.n * d.
 __device__ __forceinline__ void exit_candidate(ExitChoice& e, double n, double d, int k) {
-  // d < 0, so d * e.d has the sign opposite to e.d's: n/d < e.n/e.d is
-  // n * e.d > e.n * d when e.d > 0, and n * e.d < e.n * d when e.d < 0
   const double l = n * e.d, r = e.n * d;
-  const bool towards = d < 0;
-  const bool lt = e.d > 0 ? l > r : l < r;
-  e.odd = e.odd || (towards && !(e.d > 0 || e.d < 0));
-  const bool take = towards && lt;
+  const bool take = (d < 0) && (l < r);
   e.n = take ? n : e.n;
   e.d = take ? d : e.d;
   e.i = take ? k : e.i;
@@ -125,11 +122,18 @@ __device__ __forceinline__ void exit_divided(ExitChoice& e, double& t1, double n
 
 __device__ __forceinline__ ExitChoice earliest_exit(double n0, double d0, double n1, double d1, double n2,
                                                     double d2, double n3, double d3, double n4, double d4) {
-  ExitChoice e{n0, d0, 0, false};
+  // the end's time n0 / d0 (d0 > 0 while the neutron moves forward) enters
+  // as (-n0) / (-d0), so that e.d < 0 throughout (negation is exact, and the
+  // products and their comparison only change sign); a taken wall has d < 0
+  const bool fwd = d0 > 0;
+  ExitChoice e{fwd ? -n0 : n0, fwd ? -d0 : d0, 0, false};
   exit_candidate(e, n1, d1, 1);
   exit_candidate(e, n2, d2, 2);
   exit_candidate(e, n3, d3, 3);
   exit_candidate(e, n4, d4, 4);
+  // a zero or NaN e.d met a candidate moving towards its wall: e.d is then
+  // still d0 (nothing can be taken before the first such candidate)
+  e.odd = !(d0 > 0 || d0 < 0) && (d1 < 0 || d2 < 0 || d3 < 0 || d4 < 0);
   if (e.odd) {
     double t1 = n0 / d0;
     e.n = n0;
