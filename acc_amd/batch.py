@@ -8,9 +8,11 @@ takes instruments that share one chain structure (same component types,
 shapes, kernels, table sizes and histogram sizes; any parameter values) and
 runs them all in one persistent kernel: workgroups pull (instance, block of
 streams) work items from one atomic counter, so small and large instances
-share the chip.  A work item is one workgroup's worth of streams, or eight
+share the chip.  A work item is one workgroup's worth of streams, or sixteen
 for chains whose later stages draw random numbers (their workgroups refill
-stream slots from the item, so the queue is drained once per item).  Instance k with ncounts[k] neutrons uses exactly the
+stream slots from the item, so the queue is drained once per item).  Items
+are handed out longest first (instances with more neutrons per stream
+first), so that the launch ends on short items.  Instance k with ncounts[k] neutrons uses exactly the
 partition and random streams a separate `run_instrument(instr_k, ncounts[k],
 seed=seed)` would, so each instance's histograms are identical (up to the
 order of atomic additions) to its one-by-one run.  The launch (stream

@@ -56,10 +56,14 @@ struct LaunchArgs {
 struct BatchArgs {
   LaunchArgs base;           // offsets, rng, gate; P/H of instance 0
   int64_t pstride, hstride;  // per-instance strides of the parameter blob and histograms
-  const int64_t* item0;      // (ninst + 1) first work item of each instance (prefix sums)
+  // The instances in work order (the longest work items first): position j
+  // is instance inst_index[j]; item0, inst_n, inst_nthreads and inst_npt are
+  // listed in that order.
+  const int64_t* item0;      // (ninst + 1) first work item of each position (prefix sums)
   const int64_t* inst_n;     // (ninst) neutrons
   const int64_t* inst_nthreads;
   const int64_t* inst_npt;
+  const int64_t* inst_index;  // (ninst)
   unsigned long long* inst_processed;  // (ninst)
   unsigned long long* work;  // work-item counter (zeroed before the launch)
   int32_t ninst;

@@ -7,6 +7,7 @@
 // mcv_components.h / mcv_samples.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1138,7 +1139,7 @@ struct mcv_batch {
   double* d_hist = nullptr;
   uint64_t* d_rng = nullptr;
   int64_t rng_cap = 0;
-  int64_t* d_meta = nullptr;                 // item0 (ninst+1), n, nthreads, npt (ninst each)
+  int64_t* d_meta = nullptr;                 // item0 (ninst+1), n, nthreads, npt, instance (ninst each)
   unsigned long long* d_counters = nullptr;  // processed (ninst), work item counter, stream pool
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr, fn_batch = nullptr;
@@ -1214,7 +1215,7 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   const int64_t hn = std::max<int64_t>(hist_total * ninst, 1);
   HIPCHECK(hipMalloc(&b->d_hist, hn * sizeof(double)));
   HIPCHECK(hipMemset(b->d_hist, 0, hn * sizeof(double)));
-  HIPCHECK(hipMalloc(&b->d_meta, (size_t)(4 * ninst + 1) * sizeof(int64_t)));
+  HIPCHECK(hipMalloc(&b->d_meta, (size_t)(5 * ninst + 1) * sizeof(int64_t)));
   HIPCHECK(hipMalloc(&b->d_counters, (size_t)(ninst + 2) * sizeof(unsigned long long)));
   hipError_t e = hipModuleLoad(&b->mod, code_object);
   if (e != hipSuccess) return fail(MCV_ERR_MODULE, "hipModuleLoad(%s): %s", code_object, hipGetErrorString(e));
@@ -1271,6 +1272,7 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.inst_n = b->d_meta + b->ninst + 1;
   B.inst_nthreads = b->d_meta + 2 * b->ninst + 1;
   B.inst_npt = b->d_meta + 3 * b->ninst + 1;
+  B.inst_index = b->d_meta + 4 * b->ninst + 1;
   B.inst_processed = b->d_counters;
   B.work = b->d_counters + b->ninst;
   B.ninst = b->ninst;
@@ -1303,17 +1305,29 @@ int mcv_batch_run(mcv_batch* b, const uint64_t* ncounts, uint64_t seed, int64_t 
   if (!b || !ncounts) return fail(MCV_ERR_ARG, "bad batch arguments");
   const auto t0 = std::chrono::steady_clock::now();
   const int n = b->ninst;
-  std::vector<int64_t> meta(4 * n + 1, 0);
+  // work order: instances with more neutrons per stream (longer work items)
+  // first, so that the launch ends on short items and no workgroup is left
+  // with a long one while the others run dry
+  std::vector<int64_t> nt_of(n), npt_of(n);
+  std::vector<int> order(n);
+  for (int k = 0; k < n; ++k) {
+    int64_t nb;
+    partition(ncounts[k], ntotalthreads, tpb, nb, nt_of[k], npt_of[k]);
+    order[k] = k;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return npt_of[x] > npt_of[y]; });
+  std::vector<int64_t> meta(5 * n + 1, 0);
   int64_t nthreads_max = 0;
   uint64_t total_n = 0;
-  for (int k = 0; k < n; ++k) {
-    int64_t nb, nt, npt;
-    partition(ncounts[k], ntotalthreads, tpb, nb, nt, npt);
+  for (int j = 0; j < n; ++j) {
+    const int k = order[j];
+    const int64_t nt = nt_of[k], npt = npt_of[k];
     const int64_t per_item = b->item_streams;
-    meta[k + 1] = meta[k] + (nt + per_item - 1) / per_item;
-    meta[n + 1 + k] = (int64_t)ncounts[k];
-    meta[2 * n + 1 + k] = nt;
-    meta[3 * n + 1 + k] = npt;
+    meta[j + 1] = meta[j] + (nt + per_item - 1) / per_item;
+    meta[n + 1 + j] = (int64_t)ncounts[k];
+    meta[2 * n + 1 + j] = nt;
+    meta[3 * n + 1 + j] = npt;
+    meta[4 * n + 1 + j] = k;
     nthreads_max = std::max(nthreads_max, nt);
     total_n += ncounts[k];
   }
