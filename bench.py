@@ -192,7 +192,7 @@ CONFIG_TABLE = (("c1", int(1e7)), ("c2", int(1e8)), ("c2s", int(1e8)), ("c3", in
 
 
 def config_table(steps=3, cpu_seconds=5.0, falg_neutrons=100000):
-    """The survey's configurations side by side: fused-kernel time (HIP events)
+    """The benchmark configurations side by side: fused-kernel time (HIP events)
     and wall time per run at the reference partition, neutrons/s, F_alg and
     the FP64 fraction, and the CPU oracle on all host cores and on one.
     c2 and c2s run at their autotuned shape, c1 and c3 at the default one."""
@@ -281,7 +281,7 @@ def main():
         return 0
     if a.gpus != 1:
         print("bench.py measures one GPU: the propagate path runs on a single MI355X here "
-              "(multi-GPU sharding by stream ranges is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+              "(sharding by stream ranges over several GPUs is not built); use --gpus 1", file=sys.stderr)
         return 2
 
     from acc_amd import configs
