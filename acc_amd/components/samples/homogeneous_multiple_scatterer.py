@@ -42,8 +42,11 @@ def factory(
             "mcv::hms_scatterM<{shape}, {kind}, {M}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}], out)",
             stochastic=True, preamble=SampleBase._shape_preamble)
         # the same scatterer handing each output to the callable {emit} as it
-        # is produced (for chains whose downstream components draw nothing)
-        propagate_emit = "mcv::hms_scatter<{shape}, {kind}, {M}>(rng, {n}, P + A.off[{i}], P + A.doff[{i}], {emit})"
+        # is produced (for chains whose downstream components draw nothing);
+        # {checked}: true where the kernel has already found that the neutron
+        # is not exiting the shape (the scatterer's first test)
+        propagate_emit = ("mcv::hms_scatter<{shape}, {kind}, {M}, {checked}>"
+                          "(rng, {n}, P + A.off[{i}], P + A.doff[{i}], {emit})")
 
         def _render_context(self, i):
             ctx = SampleBase._render_context(self, i)

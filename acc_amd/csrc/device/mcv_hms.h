@@ -12,9 +12,10 @@
 // kernel: as separate functions their frames and by-reference neutrons went
 // through scratch at every call (c3, 1e8 neutrons: 75 ms with both out of
 // line, 70 with the first-encounter loop inlined, 60 with both; 48.2 -> 43.4
-// ms later with the output lists gone).  Keeping each list's first entry in
-// registers measured slower (52 ms: the register file is full, and spills
-// replace the scratch traffic).
+// ms later with the output lists gone, 36 ms with each root list and location
+// computed once where the reference computes it twice).  Keeping each list's
+// first entry in registers measured slower (52 ms: the register file is full,
+// and spills replace the scratch traffic).
 #pragma once
 #include "mcv_samples.h"
 
@@ -47,11 +48,17 @@ __device__ __forceinline__ int locate0(const double* S, double x, double y, doub
   return Shape::locate(S, x, y, z);
 }
 
+// A location not computed yet (the reference's OUTSIDE, ONBORDER and INSIDE
+// are 1, 0 and -1).  The helpers below that take `loc` skip the reference's
+// first locate() of the neutron when the caller already knows its result for
+// the same position: locate() is a pure function of the position.
+constexpr int LOC_UNKNOWN = 2;
+
 // propagation.py:24-50
 template <class Shape>
-__device__ __forceinline__ bool is_exiting(const double* S, const Neutron& n) {
+__device__ __forceinline__ bool is_exiting(const double* S, const Neutron& n, int loc = LOC_UNKNOWN) {
   constexpr int K = Shape::K;
-  const int loc = locate0<Shape>(S, n.x, n.y, n.z);
+  if (loc == LOC_UNKNOWN) loc = locate0<Shape>(S, n.x, n.y, n.z);
   if (loc == INSIDE) return false;
   double ts[K];
   const int N = forward_intersect<Shape>(S, n.x, n.y, n.z, n.vx, n.vy, n.vz, ts);
@@ -86,10 +93,11 @@ __device__ __forceinline__ void propagate_out(const double* S, Neutron& n) {
 
 // propagation.py:61-105
 template <class Shape>
-__device__ __forceinline__ void propagate_to_next_incident_surface(const double* S, Neutron& n) {
+__device__ __forceinline__ void propagate_to_next_incident_surface(const double* S, Neutron& n,
+                                                                   int loc = LOC_UNKNOWN) {
   constexpr int K = Shape::K;
   const double x = n.x, y = n.y, z = n.z, vx = n.vx, vy = n.vy, vz = n.vz;
-  const int loc = locate0<Shape>(S, x, y, z);
+  if (loc == LOC_UNKNOWN) loc = locate0<Shape>(S, x, y, z);
   if (loc == INSIDE) return;
   double ts[K];
   const int N = forward_intersect<Shape>(S, x, y, z, vx, vy, vz, ts);
@@ -184,35 +192,53 @@ __device__ __forceinline__ double atten_coeff(const double* P, const Neutron& n)
   return P[1] + absorption_coeff(P, n);
 }
 
-// :69-79
+// :69-79, for a neutron inside the shape (its only use).  Inside, the
+// reference's tof_before_first_exit is the first forward root (0 without
+// one) and propagate_to_next_exiting_surface moves by that same first root:
+// one intersection serves both.
 template <class Shape>
-__device__ __forceinline__ void propagate_to_next_exiting_surface_with_attenuation(const double* P, const double* D,
-                                                                                   Neutron& n) {
+__device__ __forceinline__ void propagate_inside_to_exiting_surface_with_attenuation(const double* P,
+                                                                                    const double* D, Neutron& n) {
   const double vx = n.vx, vy = n.vy, vz = n.vz;
-  const double tof = tof_before_first_exit<Shape>(D, n);
+  double ts[Shape::K];
+  const int N = forward_intersect<Shape>(D, n.x, n.y, n.z, vx, vy, vz, ts);
+  const double tof = N == 0 ? 0. : ts[0];
   const double v = sqrt(vx * vx + vy * vy + vz * vz);
   const double distance = tof * v;
   const double atten = atten_coeff(P, n);
   const double transmission_prob = exp(-atten * distance);
-  propagate_to_next_exiting_surface<Shape>(D, n);
+  prop_dt_inplace(n, ts[0]);
   n.p *= transmission_prob;
 }
 
 // :81-121; out[0] transmitted, out[1] scattered
+// (`loc`: the neutron's location if the caller knows it.)
 template <class Shape, int KIND>
-__device__ __forceinline__ int interactM1(Rng& rng, const double* P, const double* D, Neutron* out, Neutron& n) {
+__device__ __forceinline__ int interactM1(Rng& rng, const double* P, const double* D, Neutron* out, Neutron& n,
+                                          int loc = LOC_UNKNOWN) {
   const double vx = n.vx, vy = n.vy, vz = n.vz;
-  const int loc = locate0<Shape>(D, n.x, n.y, n.z);
-  if (loc != INSIDE) propagate_to_next_incident_surface<Shape>(D, n);
-  const double tof = tof_before_first_exit<Shape>(D, n);
+  if (loc == LOC_UNKNOWN) loc = locate0<Shape>(D, n.x, n.y, n.z);
+  double tof;
+  if (loc == INSIDE) {  // the neutron stays put: tof_before_first_exit is its first root
+    double ts[Shape::K];
+    const int N = forward_intersect<Shape>(D, n.x, n.y, n.z, vx, vy, vz, ts);
+    tof = N == 0 ? 0. : ts[0];
+  } else {
+    propagate_to_next_incident_surface<Shape>(D, n, loc);
+    tof = tof_before_first_exit<Shape>(D, n);
+  }
   if (tof == 0.) return 0;
   const double v = sqrt(vx * vx + vy * vy + vz * vz);
   const double distance = tof * v;
   const double sigma = P[1];
   const double mu = absorption_coeff(P, n);
   const double transmission_prob = exp(-(mu + sigma) * distance);
+  // propagate_to_next_exiting_surface(out[0]) moves it by exactly tof here:
+  // the neutron is inside or on the border (else tof is 0), and for both the
+  // reference computes the same roots and the same midpoint tests as
+  // tof_before_first_exit did
   out[0] = n;
-  propagate_to_next_exiting_surface<Shape>(D, out[0]);
+  prop_dt_inplace(out[0], tof);
   out[0].p *= transmission_prob;
   // the absorption branch writes out[1] and is overwritten below; the kernels'
   // absorb() is a no-op (acc/kernels/__init__.py:490-492)
@@ -249,9 +275,12 @@ __device__ __forceinline__ Neutron load_s(const NeutronS& d, double s1, double s
   return n;
 }
 
-// :128-178.  Each output is handed to emit(const Neutron&) in the order in
-// which the reference stores it into its output list (no list is kept); the
-// return value is the reference's output count.
+// :128-178.  Each output is handed to emit(const Neutron&, int loc) in the
+// order in which the reference stores it into its output list (no list is
+// kept), with its location when known (LOC_UNKNOWN otherwise); the return
+// value is the reference's output count.  Entries of the list of neutrons to
+// scatter are inside the shape (that is the test that put them there), except
+// the incident neutron.
 template <class Shape, int KIND, int M, class Emit>
 __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const double* D, int out_len,
                                                const Neutron& n, Emit&& emit) {
@@ -270,15 +299,16 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
     int s2 = 0;
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
       Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
-      const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, cur);
+      const int ns1 = interactM1<Shape, KIND>(rng, P, D, scattered, cur, iloop > 0 ? INSIDE : LOC_UNKNOWN);
       // at most two outputs: constant indices keep them in registers
 #pragma unroll
       for (int is = 0; is < 2; ++is) {
         if (is >= ns1) break;
         const Neutron& sc1 = scattered[is];
         if (sc1.p < min_prob) continue;
-        if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) != INSIDE) {
-          emit(sc1);
+        const int loc = locate0<Shape>(D, sc1.x, sc1.y, sc1.z);
+        if (loc != INSIDE) {
+          emit(sc1, loc);
           out_index += 1;
           if (out_index >= lim) return out_index;
           continue;
@@ -297,7 +327,7 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
   }
   for (int i = 0; i < N_tbs; ++i) {
     if (out_index >= lim) break;
-    emit(load_s(tbs[i], n.s1, n.s2));
+    emit(load_s(tbs[i], n.s1, n.s2), loops > 0 ? INSIDE : LOC_UNKNOWN);
     out_index += 1;
   }
   return out_index;
@@ -312,10 +342,12 @@ __device__ __forceinline__ int interactM_path1(Rng& rng, const double* P, const 
 // neutrons to scatter is full, or M outputs are out), keeps every draw and
 // every output.  The incident neutron is taken by value: an emit that
 // overwrites the caller's neutron does not change the spins read here.
-template <class Shape, int KIND, int M, class Emit>
+// CHECKED: the caller has already found that is_exiting(n) is false (the
+// compacted kernels test it to decide which neutrons interact).
+template <class Shape, int KIND, int M, bool CHECKED = false, class Emit>
 __device__ __forceinline__ int hms_scatter(Rng& rng, const Neutron n, const double* __restrict__ P,
                                            const double* __restrict__ D, Emit&& emit) {
-  if (is_exiting<Shape>(D, n)) {
+  if (!CHECKED && is_exiting<Shape>(D, n)) {
     emit(n);
     return 1;
   }
@@ -332,14 +364,20 @@ __device__ __forceinline__ int hms_scatter(Rng& rng, const Neutron n, const doub
     bool full = false, done = false;  // the reference's break (s2 reached M) and return (M outputs)
     for (int ineutron = 0; ineutron < N_tbs; ++ineutron) {
       const Neutron cur = load_s(tbs[ineutron], n.s1, n.s2);
-      interactM_path1<Shape, KIND, M>(rng, P, D, M, cur, [&](const Neutron& o) {
+      interactM_path1<Shape, KIND, M>(rng, P, D, M, cur, [&](const Neutron& o, int oloc) {
         if (full || done) return;
         Neutron sc1 = o;
         if (sc1.p < min_prob) return;
         if (!is_moving(sc1)) return;
-        if (locate0<Shape>(D, sc1.x, sc1.y, sc1.z) == INSIDE)
-          propagate_to_next_exiting_surface_with_attenuation<Shape>(P, D, sc1);
-        if (is_exiting<Shape>(D, sc1)) {
+        const int loc = oloc == LOC_UNKNOWN ? locate0<Shape>(D, sc1.x, sc1.y, sc1.z) : oloc;
+        bool exiting;
+        if (loc == INSIDE) {
+          propagate_inside_to_exiting_surface_with_attenuation<Shape>(P, D, sc1);
+          exiting = is_exiting<Shape>(D, sc1);
+        } else {
+          exiting = is_exiting<Shape>(D, sc1, loc);
+        }
+        if (exiting) {
           emit(sc1);
           out_index += 1;
           done = out_index >= M;

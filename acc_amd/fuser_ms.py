@@ -34,7 +34,7 @@ def _tail(comps, m, n="o"):
     return "\n      ".join(tail)
 
 
-def _scatter_and_tail(comps, m):
+def _scatter_and_tail(comps, m, checked=False):
     """Statements running the scatterer and, on each output, the downstream
     components; the first output's final state is left in `n`.  When nothing
     downstream draws, each output goes down the chain as the scatterer hands
@@ -45,7 +45,8 @@ def _scatter_and_tail(comps, m):
     M = c.max_scattered_neutrons
     tail = _tail(comps, m)
     if not any(d.propagate.stochastic for d in comps[m + 1:]):
-        call = c.propagate_emit.format(i=m, n="n", emit="emit", **c._render_context(m))
+        call = c.propagate_emit.format(i=m, n="n", emit="emit", checked="true" if checked else "false",
+                                       **c._render_context(m))
         return [f"""int k = 0;
   auto emit = [&](const mcv::Neutron& out) {{
     mcv::Neutron o = out;
@@ -93,7 +94,7 @@ def _generate_compacted(comps, m, pre, structure):
     n = o;
   }}
   return -1;""")
-    b = _scatter_and_tail(comps, m)
+    b = _scatter_and_tail(comps, m, checked=True)  # phase A queued only neutrons not exiting the shape
     a_text = "\n  ".join(a)
     b_text = "\n  ".join(b)
     src = f"""// fused instrument kernel (multiple scattering), compacted at the sample: {structure}
