@@ -91,6 +91,27 @@ def _hms_reference():
     return i
 
 
+def _hms_then_sample():
+    """A multiple-scattering sample followed by a component that draws (a
+    single-scattering sphere): the scatterer's outputs are kept in a list and
+    run down the chain afterwards, in the reference's order of draws."""
+    from .instrument import instrument
+    from .configs import _source, hollow_cylinder, isotropic_al_kernel
+    from .geometry import Sphere
+    from .components.samples.homogeneous_multiple_scatterer import factory as ms_factory
+    from .components.samples.homogeneous_single_scatterer import factory as ss_factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    HMS = ms_factory(shape=hollow_cylinder(), kernel=isotropic_al_kernel(50.0, 5.0), max_ms_loops=2,
+                     max_ms_loops_path1=2, max_scattered_neutrons=8)
+    HSS = ss_factory(shape=Sphere(0.05), kernel=isotropic_al_kernel(50.0, 5.0))
+    i = instrument("hms-sample")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(HMS('ms'), position=(0, 0, 10.0))
+    i.append(HSS('ss'), position=(0, 0, 10.3))
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 10.6))
+    return i
+
+
 def _csg_chain(xmlname, orientation=(0, 0, 0)):
     """Single-scatterer sample with a shape from the reference's geometry
     tests (tests/geometry/*.xml), for the union/intersection/difference code."""
@@ -217,7 +238,7 @@ def instruments():
         "c3s": configs.c3_single, "c3s-small": lambda: configs.c3_single(ntubes=40, npixels=16, ntof=50),
         "c5s": configs.c5_single,
         "c3": configs.c3, "c3-small": lambda: configs.c3(ntubes=40, npixels=16, ntof=50),
-        "hms-ref": _hms_reference,
+        "hms-ref": _hms_reference, "hms-sample": _hms_then_sample,
         "csg-union2": lambda: _csg_chain("union_example2.xml", (90, 0, 0)),
         "csg-intersection": lambda: _csg_chain("intersection_example1.xml", (90, 0, 0)),
         "csg-difference": lambda: _csg_chain("difference_example1.xml", (90, 0, 0)),

@@ -47,3 +47,45 @@ def test_multiple_ms_components_rejected():
     i.append(H("b"), position=(0, 0, 11))
     with pytest.raises(NotImplementedError):
         fuser.generate_source(i)
+
+
+def _body(src, fn):
+    i = src.index(fn + "(")
+    return src[i:src.index("\n  }\n", i)]
+
+
+def test_entrance_split():
+    """With gating, phase A ends at the straight guide's entrance and phase B
+    starts with its bounces; without gating phase A runs the whole guide (an
+    absorbed neutron must not run the rest).  The adjoint kernel splits the
+    same way, the guide's scores starting in phase B."""
+    from acc_amd.fuser_adj import generate_adjoint_source
+    from acc_amd import configs
+    for src in (fuser.generate_source(prebuilt.instruments()["c2"]())[0],
+                generate_adjoint_source(configs.c5())[0]):
+        a, b = _body(src, "phase_a"), _body(src, "phase_b")
+        assert "if (A.gate) {\n    if (!mcv::guide_enter(n, P + A.off[1])) return 0;\n  } else {" in a
+        assert "mcv::guide_bounces" not in a and "mcv::guide_enter" not in b
+        assert b.index("if (A.gate) { mcv::guide_bounces") < b.index("mcv::guide_tapering")
+
+
+def test_ms_outputs_streamed():
+    """A multiple-scattering chain whose downstream components draw nothing
+    hands each output down the chain as it is produced; the compacted kernel
+    tells the scatterer that phase A has found the neutron not exiting."""
+    from acc_amd import configs
+    src = fuser.generate_source(configs.c3())[0]
+    b = _body(src, "phase_b")
+    assert "mcv::hms_scatter<Shape1, 2, 10, true>(rng, n, P + A.off[1], P + A.doff[1], emit)" in b
+    assert "NeutronS out[" not in b
+    plain = fuser.generate_source(configs.c3(), compact=False)[0]
+    assert "mcv::hms_scatter<Shape1, 2, 10, false>" in plain
+
+
+def test_ms_outputs_listed_before_a_drawing_component():
+    """When a component after the scatterer draws random numbers, the outputs
+    are kept in a list and run down the chain after the scatterer, as the
+    reference does (the draw order per stream depends on it)."""
+    b = _body(fuser.generate_source(prebuilt.instruments()["hms-sample"]())[0], "phase_b")
+    assert "mcv::NeutronS out[8];" in b and "mcv::hms_scatterM<Shape1, 1, 8>" in b
+    assert b.index("hms_scatterM") < b.index("hss_propagate")

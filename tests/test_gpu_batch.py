@@ -16,8 +16,11 @@ from acc_amd.batch import BatchedScan
 pytestmark = pytest.mark.gpu
 
 SEED = 5
-# a few instances with uneven counts, one of them empty
-PLAN = [(25.0, 0.0, 30000), (60.0, 45.0, 250000), (120.0, 90.0, 0), (180.0, 170.0, 1300), (40.0, 10.0, 70001)]
+# a few instances with uneven counts, one of them empty; the last has 3 neutrons
+# per stream (2.5e6 neutrons over the default 1e6 streams), so the batch hands
+# its work items out first (longest items first)
+PLAN = [(25.0, 0.0, 30000), (60.0, 45.0, 250000), (120.0, 90.0, 0), (180.0, 170.0, 1300), (40.0, 10.0, 70001),
+        (90.0, 120.0, 2500001)]
 
 
 def _instances():
@@ -88,7 +91,7 @@ def test_batch_sequential_matches(scan_and_expected):
 def test_batch_new_counts_recapture(scan_and_expected):
     """Changing the counts after a capture re-captures; an all-zero scan runs nothing."""
     scan, _ = scan_and_expected
-    n = [1000, 0, 0, 0, 5]
+    n = [1000, 0, 0, 0, 5] + [0] * (len(PLAN) - 5)
     st = scan.run(n, seed=SEED, use_graph=True)
     assert st["processed"] == sum(n)
     st = scan.run([0] * len(PLAN), seed=SEED, use_graph=True)

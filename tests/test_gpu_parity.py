@@ -62,6 +62,8 @@ CASES = [
     ("c3-small", True, 100000, 4096),
     ("c3-small", False, 50000, 2048),
     ("hms-ref", True, 30000, 2048),
+    ("hms-sample", True, 30000, 2048),
+    ("hms-sample", False, 20000, 1024),
     ("iqe-normalizer", False, 200000, 4096),
 ]
 
