@@ -27,8 +27,10 @@ kernel_refill = False
 # launch shape of chains with a multiple-scattering sample: their scatterer
 # needs the whole register file (occupancy 1 unconstrained); a target of 2
 # waves per SIMD trades spills for latency hiding (c3, 1e8 neutrons: 78 ms at
-# 128 / 2 / 4 streams per lane vs 98 ms at 256 / compiler's choice / 4)
-ms_kernel_block = 128
+# 128 / 2 / 4 streams per lane vs 98 ms at 256 / compiler's choice / 4, when
+# the scatterer was out of line; with it inlined, 34.5 ms at 64 / 2 / 4, 36.3
+# at 128 / 2 / 4, 37.6 at 256 / 2 / 4, 43.4 at 128 / 3 / 2, 53.5 at 128 / 1 / 4)
+ms_kernel_block = 64
 ms_kernel_waves_per_eu = 2
 # launch shape of the compacted adjoint kernels (their queues carry the score
 # payload; on the c5 fit chain 64 x 4 measured fastest of 64/128/256 x 1/2/4)
