@@ -30,7 +30,7 @@ class BatchedScan:
     """Instances of one chain structure, run together on the GPU."""
 
     def __init__(self, instruments, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
-        from .fuser import generate_source, compile_source
+        from .fuser import generate_source, compile_source, resolve_shape
         if not instruments:
             raise ValueError("empty scan")
         srcs = [generate_source(instr, compact=compact, block=block, waves_per_eu=waves_per_eu,
@@ -40,9 +40,8 @@ class BatchedScan:
             if s != srcs[0]:
                 raise ValueError("instance %d has a different chain structure than instance 0" % k)
         self.code_object, self.kernel_name = compile_source(srcs[0])
-        self.block = int(block or config.kernel_block)
-        self.waves_per_eu = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
-        self.streams_per_lane = int(streams_per_lane or config.kernel_streams_per_lane)
+        self.block, self.waves_per_eu, self.streams_per_lane = resolve_shape(
+            instruments[0], block, waves_per_eu, streams_per_lane)
         self.instruments = list(instruments)
         descs, offs, rots = [], [], []
         for instr in self.instruments:

@@ -101,6 +101,23 @@ def _phase(components, lo, hi, first_transform):
     return "\n  ".join(_phase_lines(components, lo, hi, first_transform) + ["return 1;"])
 
 
+def resolve_shape(instrument, block=None, waves_per_eu=None, streams_per_lane=None):
+    """(workgroup size, occupancy target or None, streams per lane) of the
+    fused kernel, the given values first, then the chain's defaults:
+    config.ms_kernel_block / ms_kernel_waves_per_eu for chains with a
+    multiple-scattering sample, config.kernel_block / kernel_waves_per_eu
+    otherwise, and config.kernel_streams_per_lane."""
+    from . import config
+    if any(c.is_multiplescattering for c in instrument.components):
+        block = block or config.ms_kernel_block
+        if waves_per_eu is None:
+            waves_per_eu = config.ms_kernel_waves_per_eu
+    block = int(block or config.kernel_block)
+    wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
+    spl = int(streams_per_lane or config.kernel_streams_per_lane)
+    return block, wpe, spl
+
+
 def generate_source(instrument, compact=True, block=None, waves_per_eu=None, streams_per_lane=None, refill=None):
     """(source text, structural description) of the fused kernel, for
     workgroups of `block` threads (default config.kernel_block), if given an
@@ -111,15 +128,10 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
     multiple-scattering sample default to config.ms_kernel_block and
     config.ms_kernel_waves_per_eu instead."""
     from . import config
-    if any(c.is_multiplescattering for c in instrument.components):
-        block = block or config.ms_kernel_block
-        if waves_per_eu is None:
-            waves_per_eu = config.ms_kernel_waves_per_eu
-    block = int(block or config.kernel_block)
+    block, wpe, spl = resolve_shape(instrument, block, waves_per_eu, streams_per_lane)
     if block % 64 or not 64 <= block <= 1024:
         raise ValueError("workgroup size must be a multiple of 64 in [64, 1024], got %d" % block)
     refill = config.kernel_refill if refill is None else refill
-    wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
     defs = "#define MCV_BLOCK %d\n" % block
     if wpe:
         if not 1 <= int(wpe) <= 8:
@@ -127,7 +139,6 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
         defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
     src, structure = _generate(instrument, compact, refill)
     if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src or "MCV_DEFINE_KERNELS_REFILL_RNG" in src:
-        spl = int(streams_per_lane or config.kernel_streams_per_lane)
         if not 1 <= spl <= 16:
             raise ValueError("streams per lane must be in [1, 16], got %d" % spl)
         defs += "#define MCV_SPL %d\n" % spl

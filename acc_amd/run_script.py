@@ -68,18 +68,16 @@ def compile(script, compiled_script=None, **kwds):
 
 
 def get_handle(instrument, block=None, waves_per_eu=None, streams_per_lane=None):
-    """The device handle of an instrument for workgroups of `block` threads
-    (default config.kernel_block), an optional occupancy target and, for
-    stream-carrying compacted chains, `streams_per_lane`; built on first use,
-    then reused."""
-    block = int(block or config.kernel_block)
-    wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
-    spl = int(streams_per_lane or config.kernel_streams_per_lane)
+    """The device handle of an instrument for workgroups of `block` threads,
+    an optional occupancy target and, for stream-carrying compacted chains,
+    `streams_per_lane` (each defaulting to the chain's, fuser.resolve_shape);
+    built on first use, then reused."""
+    from .fuser import fuse, resolve_shape
+    block, wpe, spl = resolve_shape(instrument, block, waves_per_eu, streams_per_lane)
     handles = instrument.__dict__.setdefault("_mcv_handles", {})
     h = handles.get((block, wpe, spl))
     if h is not None:
         return h
-    from .fuser import fuse
     code_object, kernel_name = fuse(instrument, block=block, waves_per_eu=wpe, streams_per_lane=spl)
     offsets, rotmats = calcTransformations(instrument)
     descs = [c.mcv_desc() for c in instrument.components]
