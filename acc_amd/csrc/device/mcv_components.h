@@ -269,7 +269,8 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // lhhalf): [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf *
 // l_seg (the reference's products in cv1 and ch1), [6] zr = seg * l_seg, [7]
 // zr + l_seg, [8] l_seg^2 + ww^2 and [9] its sqrt, [10] l_seg^2 + hh^2 and
-// [11] its sqrt (the reference's nlen2 and sqrt(nlen2)), [12] h1, [13] whalf.
+// [11] its sqrt (the reference's nlen2 and sqrt(nlen2)), [12] h1, [13] whalf,
+// [14] 1 / [8] and [15] 1 / [10] (correctly rounded, for div_recip).
 // A segment entrance reads its eight values from one 128-byte record instead
 // of rows nseg apart.
 // The reference's two nested loops (segments, then bounces inside a segment)
@@ -418,7 +419,8 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
       const bool vert = i <= 2;
       const double wh = vert ? wws : hhh;
       const double vdn = e.d;
-      const double d = 2.0 * vdn / D[16 * s.seg + (vert ? 8 : 10)];
+      const double* __restrict__ rec = D + 16 * s.seg;
+      const double d = div_recip(2.0 * vdn, rec[vert ? 8 : 10], rec[vert ? 14 : 15]);
       const double dl = d * l_seg;
       s.vx = i == 1 ? vx - dl : i == 2 ? vx + dl : vx;
       s.vy = i == 3 ? vy - dl : i == 4 ? vy + dl : vy;

@@ -62,6 +62,23 @@ __device__ __forceinline__ void prop_dt_inplace(Neutron& n, double dt) {
   n.t = n.t + dt;
 }
 
+// ------------------------------------------------------------------ division by a known reciprocal
+// a / b rounded to nearest, given y = 1 / b rounded to nearest (computed once,
+// e.g. on the host, for a divisor that many divisions share).  Markstein's
+// theorem (P. Markstein, IBM J. Res. Dev. 34, 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, "Markstein's theorem"): if y is 1 / b rounded to
+// nearest and q is within one ulp of a / b, then r = a - b q is exact (one
+// FMA) and q + r y rounded to nearest is a / b rounded to nearest.  q0 = a y
+// is within 1.5 ulp, so one correction brings it within ~0.5 ulp and the
+// second applies the theorem.  Five FP64 instructions instead of the ~11 of a
+// division; valid while a, b, a / b and the remainders stay in the normal
+// range (the guides' operands are of order 1e-3 to 1e4).
+__device__ __forceinline__ double div_recip(double a, double b, double y) {
+  const double q0 = a * y;
+  const double q1 = fma(fma(-b, q0, a), y, q0);
+  return fma(fma(-b, q1, a), y, q1);
+}
+
 // ------------------------------------------------------------------ histograms
 // Monitors own (3, nbins) float64 blocks: N, sum p, sum p^2, updated with
 // device-scope FP64 atomic adds that return nothing (the wave does not wait

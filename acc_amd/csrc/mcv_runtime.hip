@@ -257,7 +257,8 @@ void taper_record(double l_seg, int64_t seg, double w1, double h1, double ww, do
   r[11] = std::sqrt(r[10]);
   r[12] = h1;
   r[13] = whalf;
-  for (int k = 14; k < kTaperRec; ++k) r[k] = 0;
+  r[14] = 1.0 / r[8];  // for the bounce's 2 v.n / nlen2 (div_recip)
+  r[15] = 1.0 / r[10];
 }
 
 void taper_records(double l_seg, int64_t nseg, const double* T, double* D) {  // T: 10 x nseg
