@@ -127,33 +127,46 @@ const std::vector<uint64_t>& jump_matrices() {
   return flat;
 }
 
-__global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restrict__ mats, int nmats, uint64_t z,
-                                                        int64_t n, uint64_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint64_t a = z, b = z;
-  for (int k = 0; k < nmats; ++k) {
-    if ((i >> k) & 1) {
-      const uint64_t* __restrict__ M = mats + (size_t)k * 256;
-      uint64_t oa = 0, ob = 0;
+// state <- M state over GF(2), M one of the jump tables (128 x 128 bits).
+__device__ __forceinline__ void gf2_apply(const uint64_t* __restrict__ M, uint64_t& a, uint64_t& b) {
+  uint64_t oa = 0, ob = 0;
 #pragma unroll 8
-      for (int bit = 0; bit < 64; ++bit) {
-        const uint64_t m = 0ull - ((a >> bit) & 1ull);
-        oa ^= M[2 * bit] & m;
-        ob ^= M[2 * bit + 1] & m;
-      }
-#pragma unroll 8
-      for (int bit = 0; bit < 64; ++bit) {
-        const uint64_t m = 0ull - ((b >> bit) & 1ull);
-        oa ^= M[128 + 2 * bit] & m;
-        ob ^= M[128 + 2 * bit + 1] & m;
-      }
-      a = oa;
-      b = ob;
-    }
+  for (int bit = 0; bit < 64; ++bit) {
+    const uint64_t m = 0ull - ((a >> bit) & 1ull);
+    oa ^= M[2 * bit] & m;
+    ob ^= M[2 * bit + 1] & m;
   }
-  out[2 * i] = a;
-  out[2 * i + 1] = b;
+#pragma unroll 8
+  for (int bit = 0; bit < 64; ++bit) {
+    const uint64_t m = 0ull - ((b >> bit) & 1ull);
+    oa ^= M[128 + 2 * bit] & m;
+    ob ^= M[128 + 2 * bit + 1] & m;
+  }
+  a = oa;
+  b = ob;
+}
+
+// Stream i's state is J^i applied to the seed state (J: the 2^64-step jump,
+// mats[k] = J^(2^k)).  Thread t computes stream t from the set bits of t, then
+// streams t + 2^k, t + 2 * 2^k, ... (kSeedChunk of them at most), each one
+// product with J^(2^k) after the previous: most streams cost one matrix
+// product instead of one per set bit of i, and a wavefront writes consecutive
+// streams.  (Powers of J commute, so any order of the products gives J^i.)
+constexpr int kSeedChunk = 8;
+
+__global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restrict__ mats, int k, uint64_t z,
+                                                        int64_t n, uint64_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step = int64_t(1) << k;
+  if (t >= step || t >= n) return;
+  uint64_t a = z, b = z;
+  for (int bit = 0; bit < k; ++bit)
+    if ((t >> bit) & 1) gf2_apply(mats + (size_t)bit * 256, a, b);
+  for (int64_t i = t; i < n; i += step) {
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+    if (i + step < n) gf2_apply(mats + (size_t)k * 256, a, b);
+  }
 }
 
 // Per-device state shared by every handle: the jump tables and one stream.
@@ -201,11 +214,13 @@ int seed_streams(hipStream_t s, uint64_t seed, int64_t n, uint64_t* d_out) {
   DeviceCtx* ctx = nullptr;
   int rc = device_ctx(&ctx);
   if (rc) return rc;
-  int nm = nbits_for(n);
-  if (nm > kJumpMats) return fail(MCV_ERR_ARG, "too many streams (%lld)", (long long)n);
+  if (nbits_for(n) > kJumpMats) return fail(MCV_ERR_ARG, "too many streams (%lld)", (long long)n);
+  // threads 0 .. 2^k - 1, each seeding up to kSeedChunk streams 2^k apart
+  const int k = nbits_for((n + kSeedChunk - 1) / kSeedChunk);
+  const int64_t threads = std::min<int64_t>(int64_t(1) << k, n);
   const int bs = 256;
-  const int64_t grid = (n + bs - 1) / bs;
-  hipLaunchKernelGGL(mcv_seed_streams, dim3((unsigned)grid), dim3(bs), 0, s, ctx->d_mats, nm, splitmix64(seed), n,
+  const int64_t grid = (threads + bs - 1) / bs;
+  hipLaunchKernelGGL(mcv_seed_streams, dim3((unsigned)grid), dim3(bs), 0, s, ctx->d_mats, k, splitmix64(seed), n,
                      d_out);
   HIPCHECK(hipGetLastError());
   return MCV_OK;
@@ -236,7 +251,8 @@ void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, .
 // [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf * l_seg
 // (the products of cv1 and ch1, guide_tapering.py:202-204), [6] zr = seg *
 // l_seg and [7] zr + l_seg (:167, :205), [8] nlen2 and [9] its sqrt for the
-// vertical mirrors, [10] and [11] for the horizontal ones, [12] h1, [13] whalf.
+// vertical mirrors, [10] and [11] for the horizontal ones, [12] h1, [13] whalf,
+// [14] 1 / [8] and [15] 1 / [10] (correctly rounded).
 constexpr int64_t kTaperRec = 16;
 
 void taper_record(double l_seg, int64_t seg, double w1, double h1, double ww, double hh, double whalf, double hhalf,
