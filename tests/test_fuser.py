@@ -89,3 +89,17 @@ def test_ms_outputs_listed_before_a_drawing_component():
     b = _body(fuser.generate_source(prebuilt.instruments()["hms-sample"]())[0], "phase_b")
     assert "mcv::NeutronS out[8];" in b and "mcv::hms_scatterM<Shape1, 1, 8>" in b
     assert b.index("hms_scatterM") < b.index("hss_propagate")
+
+
+def test_launch_defaults_resolved_per_chain():
+    """Multiple-scattering chains take their own default launch shape (one
+    wavefront per workgroup, 2 waves per SIMD); the other chains the general
+    one; given values win.  run_script.get_handle resolves its shape here."""
+    from acc_amd import config, configs
+    ms = configs.c3(ntubes=40, npixels=16, ntof=50)
+    assert fuser.resolve_shape(ms) == (config.ms_kernel_block, config.ms_kernel_waves_per_eu,
+                                       config.kernel_streams_per_lane)
+    assert fuser.resolve_shape(prebuilt.instruments()["c2"]()) == (config.kernel_block, config.kernel_waves_per_eu,
+                                                                 config.kernel_streams_per_lane)
+    assert fuser.resolve_shape(ms, block=256, waves_per_eu=3, streams_per_lane=2) == (256, 3, 2)
+    assert "#define MCV_BLOCK %d" % config.ms_kernel_block in fuser.generate_source(ms)[0]
