@@ -122,18 +122,16 @@ __device__ __forceinline__ void exit_divided(ExitChoice& e, double& t1, double n
 
 __device__ __forceinline__ ExitChoice earliest_exit(double n0, double d0, double n1, double d1, double n2,
                                                     double d2, double n3, double d3, double n4, double d4) {
-  // the end's time n0 / d0 (d0 > 0 while the neutron moves forward) enters
-  // as (-n0) / (-d0), so that e.d < 0 throughout (negation is exact, and the
-  // products and their comparison only change sign); a taken wall has d < 0
-  const bool fwd = d0 > 0;
-  ExitChoice e{fwd ? -n0 : n0, fwd ? -d0 : d0, 0, false};
+  // the end's time n0 / d0 enters as (-n0) / (-d0), so that e.d < 0
+  // throughout (negation is exact, and the products and their comparison only
+  // change sign); a taken wall has d < 0.  A neutron not moving forward (d0
+  // <= 0 or NaN) with a wall ahead takes the reference's divided chain.
+  ExitChoice e{-n0, -d0, 0, false};
   exit_candidate(e, n1, d1, 1);
   exit_candidate(e, n2, d2, 2);
   exit_candidate(e, n3, d3, 3);
   exit_candidate(e, n4, d4, 4);
-  // a zero or NaN e.d met a candidate moving towards its wall: e.d is then
-  // still d0 (nothing can be taken before the first such candidate)
-  e.odd = !(d0 > 0 || d0 < 0) && (d1 < 0 || d2 < 0 || d3 < 0 || d4 < 0);
+  e.odd = !(d0 > 0) && (d1 < 0 || d2 < 0 || d3 < 0 || d4 < 0);
   if (e.odd) {
     double t1 = n0 / d0;
     e.n = n0;
