@@ -103,3 +103,13 @@ def test_launch_defaults_resolved_per_chain():
                                                                  config.kernel_streams_per_lane)
     assert fuser.resolve_shape(ms, block=256, waves_per_eu=3, streams_per_lane=2) == (256, 3, 2)
     assert "#define MCV_BLOCK %d" % config.ms_kernel_block in fuser.generate_source(ms)[0]
+
+
+def test_scheduler_flags_per_chain():
+    """Guide chains compile with the max-ILP scheduler, multiple-scattering
+    chains with the default one (measured faster for each)."""
+    from acc_amd import configs
+    c2 = fuser.compile_flags(fuser.generate_source(prebuilt.instruments()["c2"]())[0])
+    c3 = fuser.compile_flags(fuser.generate_source(configs.c3())[0])
+    assert "-amdgpu-sched-strategy=max-ilp" in c2 and "-amdgpu-sched-strategy=max-ilp" not in c3
+    assert all(f in c3 for f in fuser.HIPCC_FLAGS) and "-ffp-contract=off" in c2
