@@ -89,8 +89,9 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
 // execute whenever any of its lanes takes the branch).  The winning wall's
 // time is divided afterwards from the same operands as in the reference, so
 // a path is bit for bit the reference's unless two times agree to the last
-// bits (a corner hit).  A zero or NaN denominator (vz = 0) sets `odd`; the
-// caller then redoes the chain with divisions.
+// bits (a corner hit).  A neutron not moving forward (vz <= 0 or NaN) with a
+// wall ahead sets `odd`, and earliest_exit then runs the reference's chain
+// with divisions.
 struct ExitChoice {
   double n, d;  // the earliest exit time so far is n / d
   int i;        // 0: the segment (guide) end; 1-4: a wall
