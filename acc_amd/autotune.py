@@ -80,7 +80,7 @@ def _spls(instrument, spls):
     """Streams-per-lane candidates: only stream-carrying compacted chains have the knob."""
     from .fuser import generate_source
     src = generate_source(instrument, block=64, waves_per_eu=0)[0]
-    carried = "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src or "MCV_DEFINE_KERNELS_REFILL_RNG" in src
+    carried = "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src
     return tuple(spls) if carried else (None,)
 
 

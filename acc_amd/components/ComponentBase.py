@@ -24,7 +24,7 @@ class DeviceFunction:
     """
 
     def __init__(self, call, stochastic=False, monitor=False, preamble=None, adjoint=None, dparams=None,
-                 binning=None, stepper=None, split=None):
+                 binning=None, split=None):
         self.call = call
         self.stochastic = stochastic
         self.monitor = monitor
@@ -36,10 +36,6 @@ class DeviceFunction:
         self.adjoint = adjoint
         self.dparams = dparams
         self.binning = binning
-        # a component that can run pass by pass (refilled phase B): dict of
-        # `state` (C++ type) and `init` / `step` / `finish` format strings
-        # ({st} = the state, {n} = the neutron, `tr` = a tracker in scope)
-        self.stepper = stepper
         # a component whose first part kills most neutrons (a guide entrance):
         # (`enter`, a bool expression that is false when the neutron dies,
         # `rest`, the statement(s) completing `call`, and `rest` in score-tracking
@@ -52,9 +48,6 @@ class DeviceFunction:
 
     def render_adjoint(self, comp, i, s, n="n"):
         return self.adjoint.format(i=i, n=n, s=s, dz="dz", **comp._render_context(i))
-
-    def render_step(self, comp, i, part, st="st", n="n"):
-        return self.stepper[part].format(i=i, n=n, st=st, **comp._render_context(i))
 
     def render_split(self, comp, i, part, n="n", s=None):
         return self.split[part].format(i=i, n=n, s=s, **comp._render_context(i))

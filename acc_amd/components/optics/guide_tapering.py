@@ -85,10 +85,7 @@ class Guide(ComponentBase):
         "mcv::guide_tapering({n}, P + A.off[{i}], P + A.doff[{i}]);",
         adjoint="mcv::guide_tapering_t({n}, P + A.off[{i}], P + A.doff[{i}], mcv::TaperScore({s}));",
         dparams=lambda comp: [("R0", 1), ("Qcx", 2), ("Qcy", 3), ("alphax", 4), ("alphay", 5),
-                              ("mx", 6), ("my", 7), ("W", 8)],
-        stepper=dict(state="mcv::TaperState", init="mcv::taper_init({st}, {n}, P + A.off[{i}]);",
-                     step="mcv::taper_step({st}, P + A.off[{i}], P + A.doff[{i}], tr)",
-                     finish="mcv::taper_finish({st}, {n});"))
+                              ("mx", 6), ("my", 7), ("W", 8)])
 
     def _mcv_desc(self):
         nseg, scalars, data = self.propagate_params

@@ -18,12 +18,6 @@ kernel_block = 256
 kernel_waves_per_eu = None
 # random streams per lane in compacted kernels whose later stages draw
 kernel_streams_per_lane = 4
-# compacted kernels whose phase B starts with a pass-by-pass component (the
-# tapered guide) can hand lanes new neutrons while others are still in that
-# component ("refilled" phase B).  Off: on the guide chains it measured slower
-# than the plain compacted kernels (it keeps a neutron, its stream and the
-# component's state live across the refill, which costs occupancy)
-kernel_refill = False
 # launch shape of chains with a multiple-scattering sample: their scatterer
 # needs the whole register file (occupancy 1 unconstrained); a target of 2
 # waves per SIMD trades spills for latency hiding (c3, 1e8 neutrons: 78 ms at

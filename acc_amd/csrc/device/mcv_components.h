@@ -278,9 +278,8 @@ __device__ __forceinline__ void beamstop(Neutron& n, const double* __restrict__ 
 // segment.  Every neutron still performs exactly the reference's operations in
 // the reference's order, but lanes of a wavefront no longer wait for each
 // other at every segment: a wave runs max(segments + bounces) passes instead
-// of the sum over segments of the per-segment maximum.  The loop is exposed as
-// init / step / finish so that a kernel can also hand a lane a new neutron as
-// soon as its previous one leaves the guide (refilled phase B).
+// of the sum over segments of the per-segment maximum (init / step / finish
+// below).
 // Reflectivity is the costliest part of a bounce (a division for q, another in
 // R and a tanh), and a wavefront would evaluate it on every pass in which any
 // of its lanes bounces.  A bounce therefore only records its wall's v.n, its

@@ -231,7 +231,6 @@ struct QueueCore {
   double x[CAP], y[CAP], z[CAP], vx[CAP], vy[CAP], vz[CAP], t[CAP], p[CAP];
   int64_t idx[CAP];
   int count;  // slot allocator
-  int pop;    // next entry a refilled phase B hands out
   int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
 };
 
@@ -254,7 +253,6 @@ struct QueueRngLDS {
   int64_t next[MCV_BLOCK * MCV_SPL], end[MCV_BLOCK * MCV_SPL];
   uint8_t returned[MCV_BLOCK * MCV_SPL];
   int count;  // slot allocator
-  int pop;    // next entry a refilled phase B hands out
   int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
 };
 
@@ -531,229 +529,6 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
   __syncthreads();
 }
 
-// ------------------------------------------------------------------ refilled phase B
-// When phase B starts with a component that runs as a pass-by-pass state
-// machine (Chain::b_begin / b_step / b_end, e.g. the tapered guide), a flush
-// hands queued neutrons to lanes as they become free.  A lane whose neutron is
-// through that component waits (done) until half of its wavefront is done or
-// idle; then the done lanes finish phase B (b_end) together and the free lanes
-// take the next queued neutrons, so the divergent begin/end code runs once per
-// half wavefront instead of on every pass.  Entries [0, hi) of the queue are
-// processed; waves take entries independently (one LDS atomic per wave).
-#ifndef MCV_REFILL_AT
-#define MCV_REFILL_AT 32  // service a wavefront when at most this many lanes are stepping
-#endif
-
-__device__ __forceinline__ int wave_take(int* pop, bool want) {
-  const unsigned lane = threadIdx.x & 63u;
-  const unsigned long long need = __ballot(want);
-  int base = 0;
-  if (lane == 0 && need) base = atomicAdd(pop, (int)__popcll(need));
-  base = __shfl(base, 0, 64);
-  return want ? base + (int)__popcll(need & ((1ull << lane) - 1ull)) : -1;
-}
-
-enum LaneState : int { LANE_IDLE = 0, LANE_ACTIVE = 1, LANE_DONE = 2 };
-
-template <class Chain>
-__device__ __forceinline__ void flush_refill(const LaunchArgs& A, QueueLDS& q, int hi) {
-  int state = LANE_IDLE;
-  bool exhausted = false;
-  typename Chain::BState st;
-  Neutron b;
-  int64_t bidx = 0;
-  while (true) {
-    if ((int)__popcll(__ballot(state == LANE_ACTIVE)) <= MCV_REFILL_AT) {
-      if (state == LANE_DONE) {
-        const int alive = Chain::b_end(b, A, st);
-        dump_neutron(A, bidx, b, alive);
-        state = LANE_IDLE;
-      }
-      if (!exhausted) {
-        const int s = wave_take(&q.pop, state == LANE_IDLE);
-        if (state == LANE_IDLE && s < hi) {
-          bidx = q_load(q, s, b);
-          Chain::b_begin(b, A, st);
-          state = LANE_ACTIVE;
-        }
-        exhausted = __ballot(state == LANE_IDLE) != 0ull;  // an idle lane found the queue empty
-      }
-      if (__ballot(state != LANE_IDLE) == 0ull) break;
-    }
-    if (state == LANE_ACTIVE && Chain::b_step(A, st)) state = LANE_DONE;
-  }
-}
-
-template <class Chain>
-__device__ __forceinline__ void flush_refill_rng(const LaunchArgs& A, QueueRngLDS& q, int hi) {
-  int state = LANE_IDLE;
-  bool exhausted = false;
-  typename Chain::BState st;
-  Neutron b;
-  Rng r;
-  int o = 0;
-  while (true) {
-    if ((int)__popcll(__ballot(state == LANE_ACTIVE)) <= MCV_REFILL_AT) {
-      if (state == LANE_DONE) {
-        const int alive = Chain::b_end(r, b, A, st);
-        dump_neutron(A, q.next[o] - 1, b, alive);
-        q.st0[o] = r.s0;  // the stream continues from here
-        q.st1[o] = r.s1;
-        q.returned[o] = 1;
-        state = LANE_IDLE;
-      }
-      if (!exhausted) {
-        const int s = wave_take(&q.pop, state == LANE_IDLE);
-        if (state == LANE_IDLE && s < hi) {
-          q_load_state(q, s, b);
-          o = q.owner[s];
-          r.s0 = q.st0[o];
-          r.s1 = q.st1[o];
-          Chain::b_begin(b, A, st);
-          state = LANE_ACTIVE;
-        }
-        exhausted = __ballot(state == LANE_IDLE) != 0ull;
-      }
-      if (__ballot(state != LANE_IDLE) == 0ull) break;
-    }
-    if (state == LANE_ACTIVE && Chain::b_step(A, st)) state = LANE_DONE;
-  }
-}
-
-// run_block_compacted with a refilled phase B over every queued neutron
-template <class Chain>
-__device__ __forceinline__ void run_block_compacted_refill(const LaunchArgs& A, int64_t tid, QueueLDS& q) {
-  const int lt = threadIdx.x;
-  lds_hist_zero(A);
-  if (lt == 0) q.count = 0;
-  Rng rng;
-  int64_t idx, end;
-  stream_range(A, tid, rng, idx, end);
-  const int64_t mine = end - idx;
-  int par = 0, cnt = 0;
-  __syncthreads();
-  while (true) {
-    bool has = false;
-    Neutron n;
-    n.s1 = 0;
-    n.s2 = 0;
-    const int64_t my = idx;
-    if (idx < end) {
-      const int r = Chain::phase_a(rng, n, A, my);
-      has = r > 0;
-      if (!has) dump_neutron(A, my, n, -r);
-      ++idx;
-    }
-    const int s = q_append(q, has);
-    if (has) q_store(q, s, n, my);
-    int n_new;
-    bool more;
-    round_totals(q, par, has, idx < end, n_new, more);
-    cnt += n_new;
-    if (cnt >= MCV_BLOCK || (!more && cnt > 0)) {
-      if (lt == 0) q.pop = 0;
-      __syncthreads();
-      flush_refill<Chain>(A, q, cnt);
-      cnt = 0;
-      __syncthreads();
-      if (lt == 0) q.count = 0;
-      __syncthreads();
-      if (!more) break;
-    } else if (!more) {
-      break;
-    }
-  }
-  count_processed(A.counter, mine);
-  lds_hist_flush(A);
-  __syncthreads();
-}
-
-// run_block_compacted_rng with a refilled phase B over every queued neutron
-template <class Chain>
-__device__ __forceinline__ void run_block_compacted_rng_refill(const LaunchArgs& A, int64_t tid, QueueRngLDS& q) {
-  const int lt = threadIdx.x;
-  constexpr unsigned ALL = (1u << MCV_SPL) - 1u;
-  lds_hist_zero(A);
-  if (lt == 0) q.count = 0;
-  unsigned busy = 0, done = 0;
-  int64_t mine = 0;
-#pragma unroll
-  for (int k = 0; k < MCV_SPL; ++k) {
-    Rng r;
-    int64_t idx, end;
-    stream_range(A, tid + (int64_t)k * MCV_BLOCK, r, idx, end);
-    const int o = lt * MCV_SPL + k;
-    q.st0[o] = r.s0;
-    q.st1[o] = r.s1;
-    q.next[o] = idx;
-    q.returned[o] = 0;
-    mine += end - idx;
-    if (idx >= end) done |= 1u << k;
-  }
-  int par = 0, cnt = 0, rr = 0;
-  __syncthreads();
-  while (true) {
-    const unsigned avail = ~(busy | done) & ALL;
-    bool has = false;
-    Neutron n;
-    n.s1 = 0;
-    n.s2 = 0;
-    int sel = -1;
-    if (avail) {
-      const unsigned rot = ((avail >> rr) | (avail << (MCV_SPL - rr))) & ALL;
-      sel = (rr + __builtin_ctz(rot)) % MCV_SPL;
-      const int o = lt * MCV_SPL + sel;
-      Rng w;
-      w.s0 = q.st0[o];
-      w.s1 = q.st1[o];
-      const int64_t my = q.next[o];
-      const int r = Chain::phase_a(w, n, A, my);
-      has = r > 0;
-      if (!has) dump_neutron(A, my, n, -r);
-      q.st0[o] = w.s0;
-      q.st1[o] = w.s1;
-      q.next[o] = my + 1;
-      int64_t s_idx, s_end;
-      Rng unused;
-      stream_range(A, tid + (int64_t)sel * MCV_BLOCK, unused, s_idx, s_end);
-      if (my + 1 >= s_end) done |= 1u << sel;
-      if (has) busy |= 1u << sel;
-      rr = (sel + 1) % MCV_SPL;
-    }
-    const int s = q_append(q, has);
-    if (has) {
-      q_store_state(q, s, n);
-      q.owner[s] = lt * MCV_SPL + sel;
-    }
-    int n_new;
-    bool can_a;
-    round_totals(q, par, has, (~(busy | done) & ALL) != 0, n_new, can_a);
-    cnt += n_new;
-    if (cnt >= MCV_BLOCK || (!can_a && cnt > 0)) {
-      if (lt == 0) q.pop = 0;
-      __syncthreads();
-      flush_refill_rng<Chain>(A, q, cnt);
-      cnt = 0;
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < MCV_SPL; ++k) {
-        const int o = lt * MCV_SPL + k;
-        if (q.returned[o]) {
-          q.returned[o] = 0;
-          busy &= ~(1u << k);
-        }
-      }
-      if (lt == 0) q.count = 0;
-      __syncthreads();
-    } else if (!can_a) {
-      break;
-    }
-  }
-  count_processed(A.counter, mine);
-  lds_hist_flush(A);
-  __syncthreads();
-}
-
 // ------------------------------------------------------------------ batched scans
 // Next (instance, stream block) for the workgroup; false when all are taken.
 __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int64_t& tid,
@@ -842,30 +617,4 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
     mcv::run_lane_process_ms<CHAIN>(A, tid);                                                  \
-  }
-
-#define MCV_DEFINE_KERNELS_REFILL(NAME, CHAIN)                                                 \
-  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
-    __shared__ mcv::QueueLDS q;                                                               \
-    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
-    mcv::run_block_compacted_refill<CHAIN>(A, tid, q);                                        \
-  }                                                                                           \
-  extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) {           \
-    __shared__ mcv::QueueLDS q;                                                               \
-    mcv::LaunchArgs A;                                                                        \
-    int64_t tid;                                                                              \
-    while (mcv::next_item(B, A, tid)) mcv::run_block_compacted_refill<CHAIN>(A, tid, q);      \
-  }
-
-#define MCV_DEFINE_KERNELS_REFILL_RNG(NAME, CHAIN)                                             \
-  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
-    __shared__ mcv::QueueRngLDS q;                                                            \
-    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x; \
-    mcv::run_block_compacted_rng_refill<CHAIN>(A, tid, q);                                    \
-  }                                                                                           \
-  extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) {           \
-    __shared__ mcv::QueueRngLDS q;                                                            \
-    mcv::LaunchArgs A;                                                                        \
-    int64_t tid;                                                                              \
-    while (mcv::next_item(B, A, tid)) mcv::run_block_compacted_rng_refill<CHAIN>(A, tid, q);  \
   }

@@ -128,7 +128,7 @@ def resolve_shape(instrument, block=None, waves_per_eu=None, streams_per_lane=No
     return block, wpe, spl
 
 
-def generate_source(instrument, compact=True, block=None, waves_per_eu=None, streams_per_lane=None, refill=None):
+def generate_source(instrument, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
     """(source text, structural description) of the fused kernel, for
     workgroups of `block` threads (default config.kernel_block), if given an
     occupancy target of `waves_per_eu` waves per SIMD (default
@@ -137,85 +137,20 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
     kernels (default config.kernel_streams_per_lane).  Chains with a
     multiple-scattering sample default to config.ms_kernel_block and
     config.ms_kernel_waves_per_eu instead."""
-    from . import config
     block, wpe, spl = resolve_shape(instrument, block, waves_per_eu, streams_per_lane)
     if block % 64 or not 64 <= block <= 1024:
         raise ValueError("workgroup size must be a multiple of 64 in [64, 1024], got %d" % block)
-    refill = config.kernel_refill if refill is None else refill
     defs = "#define MCV_BLOCK %d\n" % block
     if wpe:
         if not 1 <= int(wpe) <= 8:
             raise ValueError("waves per SIMD must be in [1, 8], got %r" % wpe)
         defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
-    src, structure = _generate(instrument, compact, refill)
-    if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src or "MCV_DEFINE_KERNELS_REFILL_RNG" in src:
+    src, structure = _generate(instrument, compact)
+    if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src:
         if not 1 <= spl <= 16:
             raise ValueError("streams per lane must be in [1, 16], got %d" % spl)
         defs += "#define MCV_SPL %d\n" % spl
     return src.replace('#include "mcv_kernel.h"', defs + '#include "mcv_kernel.h"', 1), structure
-
-
-def _generate_refill(comps, k, pre, structure):
-    """Compacted kernel whose phase B starts with a pass-by-pass component
-    (component k+1): phase B is b_begin (frame change + init), b_step (one
-    pass) and b_end (finish + the remaining components)."""
-    s = comps[k + 1]
-    stp = s.propagate.stepper
-    a = _phase(comps, 0, k + 1, False)
-    draws = phase_b_draws(comps, k)
-    rest = []
-    for i in range(k + 2, len(comps)):
-        c = comps[i]
-        rest.append(f"mcv::apply_transformation(n, P + A.xoff[{i - 1}]);")
-        rest.append(c.propagate.render(c, i))
-        rest.append("if (A.gate && !(n.p > 0)) return 0;")
-    rest.append("return 1;")
-    end_sig = "mcv::Rng& rng, mcv::Neutron& n" if draws else "mcv::Neutron& n"
-    end_void = "(void)rng; (void)H;" if draws else "(void)H;"
-    macro = "MCV_DEFINE_KERNELS_REFILL_RNG" if draws else "MCV_DEFINE_KERNELS_REFILL"
-    src = f"""// fused instrument kernel, compacted after component {k}, refilled phase B: {structure}
-#include "mcv_kernel.h"
-
-namespace {{
-{pre}
-struct Chain {{
-  using BState = {stp["state"]};
-
-  __device__ static __forceinline__ int phase_a(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
-                                                 const int64_t nidx) {{
-  const double* __restrict__ P = A.P;
-  double* __restrict__ H = A.H;
-  (void)rng; (void)H; (void)nidx;
-  {a}
-  }}
-
-  __device__ static __forceinline__ void b_begin(mcv::Neutron& n, const mcv::LaunchArgs& A, BState& st) {{
-  const double* __restrict__ P = A.P;
-  mcv::apply_transformation(n, P + A.xoff[{k}]);
-  {s.propagate.render_step(s, k + 1, "init")}
-  }}
-
-  __device__ static __forceinline__ bool b_step(const mcv::LaunchArgs& A, BState& st) {{
-  const double* __restrict__ P = A.P;
-  mcv::NoTrack tr;
-  return {s.propagate.render_step(s, k + 1, "step")};
-  }}
-
-  __device__ static __forceinline__ int b_end({end_sig}, const mcv::LaunchArgs& A, BState& st) {{
-  const double* __restrict__ P = A.P;
-  double* __restrict__ H = A.H;
-  {end_void}
-  {s.propagate.render_step(s, k + 1, "finish")}
-  if (A.gate && !(n.p > 0)) return 0;
-  {chr(10).join("  " + l for l in rest).strip()}
-  }}
-}};
-}}  // namespace
-
-{macro}(KERNEL_NAME, Chain)
-"""
-    tag = " [compacted after %d, refilled%s]" % (k, ", streams carried" if draws else "")
-    return src, structure + tag
 
 
 def _split_phases(comps, k):
@@ -243,7 +178,7 @@ def _split_phases(comps, k):
     return "\n  ".join(a), "\n  ".join(b)
 
 
-def _generate(instrument, compact, refill=True):
+def _generate(instrument, compact):
     comps = instrument.components
     if not comps:
         raise ValueError("empty instrument")
@@ -256,8 +191,6 @@ def _generate(instrument, compact, refill=True):
     structure = " -> ".join(type(c).__name__ for c in comps)
     pre = "".join(c.propagate.render_preamble(c, i) for i, c in enumerate(comps))
     k = split_index(comps) if compact else None
-    if k is not None and refill and comps[k + 1].propagate.stepper is not None:
-        return _generate_refill(comps, k, pre, structure)
     if k is not None:
         a = _phase(comps, 0, k + 1, False)
         b = _phase(comps, k + 1, len(comps), True)
@@ -350,9 +283,8 @@ def compile_source(src, verbose=False):
     return out, name
 
 
-def fuse(instrument, verbose=False, compact=True, block=None, waves_per_eu=None, streams_per_lane=None,
-         refill=None):
+def fuse(instrument, verbose=False, compact=True, block=None, waves_per_eu=None, streams_per_lane=None):
     """Compile (or fetch from the cache) the fused kernel of an instrument."""
     src, _ = generate_source(instrument, compact=compact, block=block, waves_per_eu=waves_per_eu,
-                             streams_per_lane=streams_per_lane, refill=refill)
+                             streams_per_lane=streams_per_lane)
     return compile_source(src, verbose=verbose)

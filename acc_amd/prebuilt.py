@@ -254,8 +254,6 @@ def instruments():
 
 # chains also built for every launch shape the autotuner tries
 TUNED = ("c2", "c2s", "c5")
-# chains also built with the opt-in refilled phase B (its invariance test)
-REFILLED = ("c2", "c2s")
 # multiple-scattering chains also built uncompacted (their compaction invariance test)
 UNCOMPACTED = ("c3-small", "hms-ref")
 
@@ -279,8 +277,6 @@ def build_all(verbose=False, jobs=None):
                 for spl in _spls(instr, DEFAULT_SPL):
                     srcs["%s@%d/%s/%s" % (name, block, w, spl)] = generate_source(
                         instr, block=block, waves_per_eu=w, streams_per_lane=spl)[0]
-    for name in REFILLED:
-        srcs[name + "@refill"] = generate_source(instruments()[name](), refill=True)[0]
     for name in UNCOMPACTED:
         srcs[name + "@plain"] = generate_source(instruments()[name](), compact=False)[0]
     unique = sorted(set(srcs.values()))

@@ -24,9 +24,7 @@ def test_split_points():
     assert fuser.split_index(c2s) == 1 and fuser.phase_b_draws(c2s, 1)  # the sample draws in phase B
     src = fuser.generate_source(I["c2s"]())[0]
     assert "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src and "#define MCV_BLOCK 256" in src
-    assert "MCV_DEFINE_KERNELS_REFILL_RNG" in fuser.generate_source(I["c2s"](), refill=True)[0]
     assert "MCV_DEFINE_KERNELS_COMPACTED(" in fuser.generate_source(I["c2"]())[0]
-    assert "MCV_DEFINE_KERNELS_REFILL(" in fuser.generate_source(I["c2"](), refill=True)[0]
 
 
 def test_structure_only_enters_the_source():

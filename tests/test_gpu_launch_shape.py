@@ -37,35 +37,6 @@ def test_block_size_invariance(name, block, spl):
         np.testing.assert_allclose(h1[i][1:], h0[i][1:], rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("name", ["c2", "c2s"])
-def test_refilled_phase_b_invariance(name):
-    """The opt-in refilled phase B computes every neutron as the compacted
-    kernel does: same final states, same histograms."""
-    from acc_amd._abi import Handle
-    from acc_amd.fuser import fuse
-    from acc_amd.instrument import calcTransformations
-    print(f"\n[launch] {name} refilled phase B", flush=True)
-    instr = prebuilt.instruments()[name]()
-    N, ntot = 300000, 20000
-    s0, a0, h0 = _run(instr, 256, N, ntot)
-    off, rot = calcTransformations(instr)
-    p, k = fuse(instr, refill=True)
-    h = Handle([c.mcv_desc() for c in instr.components], off, rot, p, k)
-    try:
-        h.set_dump(0, N)
-        h.clear()
-        assert h.run(N, 3, ntot, 512, MCV_FLAG_GATE_DEAD)["processed"] == N
-        s1, a1 = h.read_dump()
-        np.testing.assert_array_equal(a1, a0)
-        np.testing.assert_array_equal(s1, s0)
-        for i in h0:
-            h1 = h.read_monitor(i)
-            np.testing.assert_array_equal(h1[0], h0[i][0])
-            np.testing.assert_allclose(h1[1:], h0[i][1:], rtol=1e-12, atol=1e-300)
-    finally:
-        h.close()
-
-
 def test_autotune_scan_small():
     print("\n[launch] autotune scan", flush=True)
     Ei, rot = [30.0, 90.0, 150.0], [0.0, 60.0, 120.0]
