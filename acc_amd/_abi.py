@@ -56,8 +56,9 @@ EXPORTED = (
     "mcv_instr_set_params", "mcv_instr_run", "mcv_monitor_read", "mcv_monitor_clear",
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
-    "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process",
+    "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process", "mcv_scratch_fit",
 )
+ABI_VERSION = 2
 
 
 class McvCompDesc(ctypes.Structure):
@@ -82,6 +83,8 @@ class McvRunStats(ctypes.Structure):
         ("ms_rng_init", ctypes.c_double),
         ("ms_kernel", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("nchunks", ctypes.c_int64),
+        ("scratch_per_lane", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -119,7 +122,7 @@ def lib():
     L.mcv_instr_set_params.argtypes = [ctypes.c_void_p, ctypes.c_int32, dp, ctypes.c_int32]
     L.mcv_instr_run.restype = ctypes.c_int
     L.mcv_instr_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
-                                ctypes.c_int32, ctypes.c_uint32, ctypes.POINTER(McvRunStats)]
+                                ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(McvRunStats)]
     L.mcv_monitor_read.restype = ctypes.c_int
     L.mcv_monitor_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, dp]
     L.mcv_monitor_clear.restype = ctypes.c_int
@@ -129,7 +132,9 @@ def lib():
     L.mcv_instr_read_dump.restype = ctypes.c_int
     L.mcv_instr_read_dump.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_int32)]
     L.mcv_rng_states.restype = ctypes.c_int
-    L.mcv_rng_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+    L.mcv_rng_states.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+    L.mcv_scratch_fit.restype = ctypes.c_int64
+    L.mcv_scratch_fit.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]
     L.mcv_instr_destroy.restype = None
     L.mcv_instr_destroy.argtypes = [ctypes.c_void_p]
     L.mcv_batch_create.restype = ctypes.c_void_p
@@ -157,7 +162,7 @@ def lib():
     L.mcv_instr_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
                                     ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, dp,
                                     ctypes.c_int32, dp, ctypes.c_int64, dp, ctypes.POINTER(McvRunStats)]
-    if L.mcv_abi_version() != 1:
+    if L.mcv_abi_version() != ABI_VERSION:
         raise ImportError("libmcvacc.so ABI version mismatch")
     _lib = L
     return L
@@ -221,10 +226,12 @@ class Handle:
         self.hoff = list(np.cumsum([0] + [3 * nb for nb in self.nbins])[:-1])
         self.hist_total = int(3 * sum(self.nbins))
 
-    def run(self, ncount, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD):
+    def run(self, ncount, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD, buffer_size=0):
+        """mcv_instr_run; buffer_size > 0 splits the run into launches of whole
+        streams of about that many neutrons (results as one launch)."""
         st = McvRunStats()
         check(lib().mcv_instr_run(self.h, int(ncount), int(seed), int(ntotalthreads or 0),
-                                  int(threads_per_block or 0), int(flags), ctypes.byref(st)))
+                                  int(threads_per_block or 0), int(flags), int(buffer_size or 0), ctypes.byref(st)))
         return st.as_dict()
 
     def set_params(self, comp, params):
@@ -288,11 +295,17 @@ class Handle:
             pass
 
 
-def rng_states(seed, n):
-    """Device-seeded (n, 2) uint64 stream states (numba create_xoroshiro128p_states)."""
+def rng_states(seed, n, first=0):
+    """Device-seeded (n, 2) uint64 states of streams [first, first + n)
+    (numba create_xoroshiro128p_states)."""
     out = np.zeros((int(n), 2), dtype=np.uint64)
-    check(lib().mcv_rng_states(int(seed), int(n), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    check(lib().mcv_rng_states(int(seed), int(first), int(n), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
     return out
+
+
+def scratch_fit(scratch_per_lane, block, resident_groups, budget_bytes):
+    """mcv_scratch_fit: workgroups that may be resident within a scratch budget."""
+    return int(lib().mcv_scratch_fit(int(scratch_per_lane), int(block), int(resident_groups), int(budget_bytes)))
 
 
 def device_count():

@@ -273,6 +273,7 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
 
 #define MCV_DEFINE_ADJOINT_KERNEL_COMPACTED(NAME, CHAIN)                                                \
   extern "C" __device__ int mcv_stream_pool = 1;                                                       \
+  extern "C" __device__ int mcv_adjoint_nscore = CHAIN::NS;                                            \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A, const mcv::AdjointArgs J) { \
     __shared__ mcv::QueueAdjLDS<CHAIN::NX> q;                                                          \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * (blockDim.x * MCV_SPL) + threadIdx.x;      \
@@ -281,6 +282,7 @@ __device__ __forceinline__ void run_block_adjoint_rng(const LaunchArgs& A, const
   }
 
 #define MCV_DEFINE_ADJOINT_KERNEL(NAME, CHAIN)                                                          \
+  extern "C" __device__ int mcv_adjoint_nscore = CHAIN::NS;                                            \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A, const mcv::AdjointArgs J) { \
     const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;                   \
     mcv::run_lane_adjoint<CHAIN>(A, J, tid);                                                           \

@@ -130,13 +130,16 @@ __device__ __forceinline__ void count_processed(unsigned long long* counter, int
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(counter, v);
 }
 
-// Streams of one lane: [tid*npt, min((tid+1)*npt, N)), or empty.
+// Neutrons of stream tid: [tid*npt, min((tid+1)*npt, N)), or empty for a
+// stream outside this launch's range (a run split into launches by stream
+// ranges computes every stream exactly as one launch over all of them does).
 __device__ __forceinline__ void stream_range(const LaunchArgs& A, int64_t tid, Rng& rng, int64_t& idx,
                                              int64_t& end) {
   idx = end = 0;
-  if (tid < A.nthreads) {
-    rng.s0 = A.rng[2 * tid];
-    rng.s1 = A.rng[2 * tid + 1];
+  if (tid < A.nthreads && tid < A.tid_end) {
+    const int64_t r = tid - A.rng_first;
+    rng.s0 = A.rng[2 * r];
+    rng.s1 = A.rng[2 * r + 1];
     idx = tid * A.npt;
     end = idx + A.npt < A.N ? idx + A.npt : A.N;
     if (end < idx) end = idx;

@@ -15,7 +15,7 @@ namespace mcv {
 struct LaunchArgs {
   const double* P;
   double* H;
-  const uint64_t* rng;               // (nthreads, 2) stream states
+  const uint64_t* rng;               // stream states (2 words each) of streams rng_first, rng_first + 1, ...
   unsigned long long* counter;       // processed neutrons
   unsigned long long* pool;          // streams taken from the pool (persistent stream-carrying kernels)
   double* dump;                      // (dump_count, 10) final states, or null
@@ -23,6 +23,8 @@ struct LaunchArgs {
   int64_t dump_first, dump_count;
   int64_t N, nthreads, npt;          // reference partition
   int64_t tid_base;                  // first stream of this launch
+  int64_t tid_end;                   // streams of this launch: [tid_base, min(tid_end, nthreads))
+  int64_t rng_first;                 // stream whose state is rng[0..1]
   int32_t gate;                      // MCV_FLAG_GATE_DEAD
   int32_t ncomp;
   int64_t off[MCV_MAX_COMP];         // parameters of component i

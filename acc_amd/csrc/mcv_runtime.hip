@@ -146,20 +146,26 @@ __device__ __forceinline__ void gf2_apply(const uint64_t* __restrict__ M, uint64
   b = ob;
 }
 
-// Stream i's state is J^i applied to the seed state (J: the 2^64-step jump,
-// mats[k] = J^(2^k)).  Thread t computes stream t from the set bits of t, then
-// streams t + 2^k, t + 2 * 2^k, ... (kSeedChunk of them at most), each one
-// product with J^(2^k) after the previous: most streams cost one matrix
-// product instead of one per set bit of i, and a wavefront writes consecutive
-// streams.  (Powers of J commute, so any order of the products gives J^i.)
+// Stream first + i's state is J^i applied to stream `first`'s state (a0, b0)
+// (J: the 2^64-step jump, mats[k] = J^(2^k)).  Thread t computes stream
+// first + t from the set bits of t, then streams t + 2^k, t + 2 * 2^k, ...
+// (kSeedChunk of them at most), each one product with J^(2^k) after the
+// previous: most streams cost one matrix product instead of one per set bit
+// of i, and a wavefront writes consecutive streams.  (Powers of J commute, so
+// any order of the products gives J^i.)  Thread 0 also zeroes `nzero` launch
+// counters, so the launch that follows on the same stream finds them cleared
+// without a memset node.
 constexpr int kSeedChunk = 8;
 
-__global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restrict__ mats, int k, uint64_t z,
-                                                        int64_t n, uint64_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restrict__ mats, int k, uint64_t a0,
+                                                        uint64_t b0, int64_t n, uint64_t* __restrict__ out,
+                                                        unsigned long long* __restrict__ zero, int nzero) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0)
+    for (int j = 0; j < nzero; ++j) zero[j] = 0;
   const int64_t step = int64_t(1) << k;
   if (t >= step || t >= n) return;
-  uint64_t a = z, b = z;
+  uint64_t a = a0, b = b0;
   for (int bit = 0; bit < k; ++bit)
     if ((t >> bit) & 1) gf2_apply(mats + (size_t)bit * 256, a, b);
   for (int64_t i = t; i < n; i += step) {
@@ -178,7 +184,26 @@ struct DeviceCtx {
   int device = -1;
   uint64_t* d_mats = nullptr;
   hipStream_t stream = nullptr;
+  int cus = 1;                 // compute units
+  int64_t scratch_budget = 0;  // bytes of lane scratch a launch may keep resident
 };
+
+// Budget for the scratch (private segment) that the lanes of one launch keep
+// resident: MCV_SCRATCH_BUDGET_MB from the environment, else an eighth of the
+// device memory free when the device is first used.  The multiple-scattering
+// kernels hold their neutron lists in lane scratch (25 KB per lane with the
+// reference's 100-neutron lists); a launch whose resident lanes would need
+// more is narrowed (persistent grids), split into several launches (the
+// fused forward kernel), or refused with MCV_ERR_NOMEM, instead of leaving
+// the runtime to fail the dispatch.
+int64_t default_scratch_budget() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return int64_t(4) << 30;
+  }
+  return (int64_t)(free_b / 8);
+}
 
 std::mutex g_ctx_mu;
 std::deque<DeviceCtx> g_ctx;  // stable addresses across push_back
@@ -198,6 +223,8 @@ int device_ctx(DeviceCtx** out) {
   HIPCHECK(hipMalloc(&c.d_mats, flat.size() * sizeof(uint64_t)));
   HIPCHECK(hipMemcpy(c.d_mats, flat.data(), flat.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   HIPCHECK(hipStreamCreate(&c.stream));
+  HIPCHECK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
+  c.scratch_budget = default_scratch_budget();
   g_ctx.push_back(c);
   *out = &g_ctx.back();
   return MCV_OK;
@@ -209,19 +236,46 @@ int nbits_for(int64_t n) {
   return k;
 }
 
-int seed_streams(hipStream_t s, uint64_t seed, int64_t n, uint64_t* d_out) {
-  if (n <= 0) return MCV_OK;
-  DeviceCtx* ctx = nullptr;
-  int rc = device_ctx(&ctx);
-  if (rc) return rc;
-  if (nbits_for(n) > kJumpMats) return fail(MCV_ERR_ARG, "too many streams (%lld)", (long long)n);
+// State of stream `first` of seed `seed`: J^first (z, z), z = SplitMix64(seed),
+// from the host copy of the jump tables (one matrix-vector product per set bit).
+V128 stream_state(uint64_t seed, int64_t first) {
+  const uint64_t z = splitmix64(seed);
+  V128 v{z, z};
+  const auto& flat = jump_matrices();
+  for (int m = 0; m < kJumpMats; ++m) {
+    if (!((first >> m) & 1)) continue;
+    const uint64_t* M = flat.data() + (size_t)m * 256;
+    V128 o{0, 0};
+    for (int bit = 0; bit < 64; ++bit)
+      if ((v.a >> bit) & 1) {
+        o.a ^= M[2 * bit];
+        o.b ^= M[2 * bit + 1];
+      }
+    for (int bit = 0; bit < 64; ++bit)
+      if ((v.b >> bit) & 1) {
+        o.a ^= M[128 + 2 * bit];
+        o.b ^= M[128 + 2 * bit + 1];
+      }
+    v = o;
+  }
+  return v;
+}
+
+// Seeds streams [first, first + n) into d_out (n x 2 words) on stream s, and
+// zeroes nzero counters at `zero` first.  `ctx` must be the current device's.
+int seed_streams(const DeviceCtx* ctx, hipStream_t s, uint64_t seed, int64_t first, int64_t n, uint64_t* d_out,
+                 unsigned long long* zero = nullptr, int nzero = 0) {
+  if (n <= 0 && nzero <= 0) return MCV_OK;
+  if (first < 0 || nbits_for(first + std::max<int64_t>(n, 1)) > kJumpMats)
+    return fail(MCV_ERR_ARG, "too many streams (%lld)", (long long)(first + n));
+  const V128 s0 = stream_state(seed, first);
   // threads 0 .. 2^k - 1, each seeding up to kSeedChunk streams 2^k apart
-  const int k = nbits_for((n + kSeedChunk - 1) / kSeedChunk);
-  const int64_t threads = std::min<int64_t>(int64_t(1) << k, n);
+  const int k = nbits_for((std::max<int64_t>(n, 1) + kSeedChunk - 1) / kSeedChunk);
+  const int64_t threads = std::max<int64_t>(std::min<int64_t>(int64_t(1) << k, n), 1);
   const int bs = 256;
   const int64_t grid = (threads + bs - 1) / bs;
-  hipLaunchKernelGGL(mcv_seed_streams, dim3((unsigned)grid), dim3(bs), 0, s, ctx->d_mats, k, splitmix64(seed), n,
-                     d_out);
+  hipLaunchKernelGGL(mcv_seed_streams, dim3((unsigned)grid), dim3(bs), 0, s, ctx->d_mats, k, s0.a, s0.b,
+                     std::max<int64_t>(n, 0), d_out, zero, nzero);
   HIPCHECK(hipGetLastError());
   return MCV_OK;
 }
@@ -551,11 +605,17 @@ __global__ void mcv_zero_f64(double* p, int64_t n) {
 
 }  // namespace
 
+constexpr int kEvRing = 8;  // event triples of a run split into launches
+
 struct mcv_instr {
   int ncomp = 0;
   int block = 256;  // workgroup size of the code object's kernels
   int spl = 1;      // random streams per lane (a workgroup serves block * spl streams)
   int pool = 0;     // the kernel takes streams from a pool (persistent grid)
+  int64_t lane_scratch = 0;  // private segment bytes per lane of the fused kernel
+  int per_cu = 0;       // its workgroups resident per CU
+  DeviceCtx* ctx = nullptr;
+  hipEvent_t rev[3 * kEvRing] = {};
   std::vector<int> types;
   std::vector<int> nparams;  // reference-order params per component (settable)
   mcv::LaunchArgs args{};
@@ -583,6 +643,9 @@ struct mcv_instr {
   int adj_block = 256;
   int adj_spl = 1;
   int adj_pool = 0;
+  int adj_ns = -1;  // score slots of the adjoint kernel
+  int64_t adj_lane_scratch = 0;
+  int adj_per_cu = 0;
   double* d_G = nullptr;
   double* d_grad = nullptr;
   int64_t grad_cap = 0;
@@ -593,6 +656,8 @@ struct mcv_instr {
   hipFunction_t proc_fn = nullptr;
   std::string proc_key;
   int proc_block = 256;
+  int64_t proc_lane_scratch = 0;
+  int proc_per_cu = 0;
   double* d_io = nullptr;
   int64_t io_cap = 0;
   double* d_io_out = nullptr;
@@ -632,16 +697,57 @@ int module_block(hipModule_t mod, int* block, int* spl = nullptr) {
   return MCV_OK;
 }
 
-// Stream-carrying kernels with a stream pool run a persistent grid: as many
-// workgroups as the device holds at once (occupancy x CUs), at most `needed`.
-int64_t persistent_grid(hipFunction_t fn, int block, size_t lds_bytes, int64_t needed) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds_bytes) != hipSuccess ||
-      cus < 1 || per_cu < 1)
-    return needed;
-  return std::min<int64_t>(needed, (int64_t)cus * per_cu);
+// Scratch bytes per lane and resident workgroups per CU of a kernel at a
+// dynamic LDS size (per_cu: the wave limit of a CU when the occupancy query
+// fails).
+void kernel_info(hipFunction_t fn, int block, size_t lds_bytes, int64_t* per_lane, int* per_cu) {
+  int v = 0;
+  if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn) != hipSuccess) {
+    (void)hipGetLastError();
+    v = 0;
+  }
+  *per_lane = v;
+  int pc = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&pc, fn, block, lds_bytes) != hipSuccess || pc < 1) {
+    (void)hipGetLastError();
+    pc = std::max(1, 32 / std::max(1, block / 64));
+  }
+  *per_cu = pc;
+}
+
+// The scratch budget in force: MCV_SCRATCH_BUDGET_MB when set (read at every
+// launch), else the device's default.
+int64_t scratch_budget(const DeviceCtx* ctx) {
+  const char* e = std::getenv("MCV_SCRATCH_BUDGET_MB");
+  if (e && *e) return (int64_t)std::max<long long>(std::atoll(e), 0) << 20;
+  return ctx->scratch_budget;
+}
+
+// Workgroups of `block` lanes needing `per_lane` bytes each that may be
+// resident together within `budget` bytes, at most `resident`; 0 when not
+// even one fits.
+int64_t scratch_fit(int64_t per_lane, int64_t block, int64_t resident, int64_t budget) {
+  if (resident <= 0) return 0;
+  if (per_lane <= 0) return resident;
+  return std::min<int64_t>(resident, budget / (per_lane * block));
+}
+
+// Grid of a launch of `needed` workgroups: persistent kernels (streams taken
+// from a pool) run as many as the device holds at once, and any kernel as many
+// as the scratch budget allows; 0 when not one fits (the caller fails).
+// `persistent` = false: the grid is `needed` when its resident part fits, else 0.
+int64_t launch_grid(const DeviceCtx* ctx, int64_t per_lane, int block, int per_cu, int64_t needed, bool persistent) {
+  const int64_t resident = std::min<int64_t>(needed, (int64_t)ctx->cus * per_cu);
+  const int64_t fit = scratch_fit(per_lane, block, resident, scratch_budget(ctx));
+  if (persistent) return fit;
+  return fit < resident ? 0 : needed;
+}
+
+int scratch_error(int64_t per_lane, int block, const DeviceCtx* ctx) {
+  return fail(MCV_ERR_NOMEM,
+              "kernel needs %lld bytes of scratch per lane; one workgroup of %d lanes exceeds the scratch budget "
+              "of %lld bytes (MCV_SCRATCH_BUDGET_MB)",
+              (long long)per_lane, block, (long long)scratch_budget(ctx));
 }
 
 void release(mcv_instr* h) {
@@ -654,6 +760,8 @@ void release(mcv_instr* h) {
   if (h->d_dump) (void)hipFree(h->d_dump);
   if (h->d_alive) (void)hipFree(h->d_alive);
   for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : h->rev)
     if (e) (void)hipEventDestroy(e);
   if (h->mod) (void)hipModuleUnload(h->mod);
   if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
@@ -688,8 +796,10 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
     int rc = device_ctx(&ctx);
     if (rc) return rc;
     h->stream = ctx->stream;  // shared, not owned
+    h->ctx = ctx;
   }
   for (auto& e : h->ev) HIPCHECK(hipEventCreate(&e));
+  for (auto& e : h->rev) HIPCHECK(hipEventCreate(&e));
   HIPCHECK(hipMalloc(&h->d_blob, B.size() * sizeof(double)));
   HIPCHECK(hipMemcpy(h->d_blob, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMalloc(&h->d_hist, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
@@ -707,7 +817,10 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
     return fail(MCV_ERR_MODULE, "kernel %s not in %s: %s", kernel_name, code_object, hipGetErrorString(e));
   rc = module_block(h->mod, &h->block, &h->spl);
   if (rc) return rc;
-  return module_int(h->mod, "mcv_stream_pool", 0, &h->pool);
+  rc = module_int(h->mod, "mcv_stream_pool", 0, &h->pool);
+  if (rc) return rc;
+  kernel_info(h->fn, h->block, h->args.lds_total * sizeof(double), &h->lane_scratch, &h->per_cu);
+  return MCV_OK;
 }
 
 void partition(uint64_t N, int64_t ntotalthreads, int32_t tpb, int64_t& nblocks, int64_t& nthreads, int64_t& npt) {
@@ -729,6 +842,121 @@ int fold_replicas(mcv_instr* h) {
                        h->d_rep + h->args.roff[i], len, h->args.rmask[i] + 1);
     HIPCHECK(hipGetLastError());
   }
+  return MCV_OK;
+}
+
+// Streams per launch of a run: all the streams that hold neutrons, or whole
+// streams of about `buffer_size` neutrons (at least one stream) when given,
+// then as few as the scratch budget allows resident at once.  A stream is
+// never split: chunk c runs streams [c*S, (c+1)*S) of the one partition of
+// ncount, each seeded as in a single launch, so a run in chunks computes
+// every neutron exactly as one launch does (the reference's buffered engine
+// instead reseeds identical streams for every buffer).  0 on a scratch error.
+int64_t streams_per_launch(const mcv_instr* h, int64_t s_used, int64_t npt, uint64_t buffer_size) {
+  int64_t S = s_used;
+  if (buffer_size > 0) S = std::min<int64_t>(S, std::max<int64_t>(1, (int64_t)(buffer_size / (uint64_t)npt)));
+  if (!h->pool) {
+    const int64_t per_group = (int64_t)h->block * h->spl;
+    const int64_t resident = std::min<int64_t>((S + per_group - 1) / per_group, (int64_t)h->ctx->cus * h->per_cu);
+    const int64_t fit = scratch_fit(h->lane_scratch, h->block, resident, scratch_budget(h->ctx));
+    if (fit <= 0) return 0;
+    if (fit < resident) S = std::min<int64_t>(S, fit * per_group);
+  }
+  return S;
+}
+
+// One run of ncount neutrons in launches over stream ranges.  `states(S, &p)`
+// provides a device buffer for the states of S streams; `counter` holds the
+// processed count and the stream-pool counter (zeroed by the first seeding).
+template <class States>
+int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
+             uint64_t buffer_size, mcv_run_stats* stats, States states, unsigned long long* counter) {
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t nblocks, nthreads, npt;
+  partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->nblocks = nblocks;
+    stats->nthreads = nthreads;
+    stats->npt = npt;
+    stats->scratch_per_lane = h->lane_scratch;
+  }
+  if (ncount == 0) return MCV_OK;
+  // streams past the last one holding neutrons run nothing
+  const int64_t s_used = std::min<int64_t>(nthreads, (int64_t)((ncount + (uint64_t)npt - 1) / (uint64_t)npt));
+  const int64_t S = streams_per_launch(h, s_used, npt, buffer_size);
+  if (S <= 0) return scratch_error(h->lane_scratch, h->block, h->ctx);
+  uint64_t* d_states = nullptr;
+  int rc = states(S, &d_states);
+  if (rc) return rc;
+  const int64_t nchunks = (s_used + S - 1) / S;
+  const int64_t per_group = (int64_t)h->block * h->spl;
+  // a persistent grid: as many workgroups as are resident at once within the scratch budget
+  const int64_t pool_grid =
+      h->pool ? launch_grid(h->ctx, h->lane_scratch, h->block, h->per_cu, (S + per_group - 1) / per_group, true) : 0;
+  if (h->pool && pool_grid <= 0) return scratch_error(h->lane_scratch, h->block, h->ctx);
+  const unsigned lds = (unsigned)(h->args.lds_total * sizeof(double));
+  double ms_rng = 0, ms_k = 0;
+  auto harvest = [&](int64_t c) -> int {
+    hipEvent_t* e = h->rev + 3 * (c % kEvRing);
+    HIPCHECK(hipEventSynchronize(e[2]));
+    float a = 0, b = 0;
+    HIPCHECK(hipEventElapsedTime(&a, e[0], e[1]));
+    HIPCHECK(hipEventElapsedTime(&b, e[1], e[2]));
+    ms_rng += a;
+    ms_k += b;
+    return MCV_OK;
+  };
+  for (int64_t c = 0; c < nchunks; ++c) {
+    if (c >= kEvRing && (rc = harvest(c - kEvRing))) return rc;
+    hipEvent_t* e = h->rev + 3 * (c % kEvRing);
+    const int64_t s0 = c * S, s1 = std::min<int64_t>(s0 + S, s_used);
+    HIPCHECK(hipEventRecord(e[0], h->stream));
+    // the first launch also zeroes the processed count; every launch its stream pool
+    rc = seed_streams(h->ctx, h->stream, seed, s0, s1 - s0, d_states, c == 0 ? counter : counter + 1, c == 0 ? 2 : 1);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(e[1], h->stream));
+    mcv::LaunchArgs A = h->args;
+    A.P = h->d_blob;
+    A.H = h->d_hist;
+    A.R = h->d_rep;
+    A.rng = d_states;
+    A.rng_first = s0;
+    A.counter = counter;
+    A.pool = h->pool ? counter + 1 : nullptr;
+    A.dump = h->d_dump;
+    A.dump_alive = h->d_alive;
+    A.dump_first = h->dump_first;
+    A.dump_count = h->d_dump ? h->dump_count : 0;
+    A.N = (int64_t)ncount;
+    A.nthreads = nthreads;
+    A.npt = npt;
+    A.tid_base = s0;
+    A.tid_end = s1;
+    A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
+    int64_t grid = (s1 - s0 + per_group - 1) / per_group;
+    if (h->pool) grid = std::min(grid, pool_grid);
+    void* kargs[] = {&A};
+    HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, (unsigned)h->block, 1, 1, lds, h->stream, kargs,
+                                   nullptr));
+    if (c + 1 == nchunks && (rc = fold_replicas(h))) return rc;
+    HIPCHECK(hipEventRecord(e[2], h->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (int64_t c = std::max<int64_t>(0, nchunks - kEvRing); c < nchunks; ++c)
+    if ((rc = harvest(c))) return rc;
+  unsigned long long processed = 0;
+  HIPCHECK(hipMemcpy(&processed, counter, sizeof(processed), hipMemcpyDeviceToHost));
+  if (stats) {
+    stats->processed = processed;
+    stats->ms_rng_init = ms_rng;
+    stats->ms_kernel = ms_k;
+    stats->nchunks = nchunks;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (processed != ncount)
+    return fail(MCV_ERR_COUNT, "Processed neutron count %llu does not match requested neutron count %llu",
+                (unsigned long long)processed, (unsigned long long)ncount);
   return MCV_OK;
 }
 
@@ -786,72 +1014,21 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
 }
 
 int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
-                  mcv_run_stats* stats) {
+                  uint64_t buffer_size, mcv_run_stats* stats) {
   if (!h) return fail(MCV_ERR_ARG, "null handle");
-  const auto t0 = std::chrono::steady_clock::now();
-  int64_t nblocks, nthreads, npt;
-  partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    stats->nblocks = nblocks;
-    stats->nthreads = nthreads;
-    stats->npt = npt;
-  }
-  if (ncount == 0) return MCV_OK;
-  if (nthreads > h->rng_cap) {
-    if (h->d_rng) HIPCHECK(hipFree(h->d_rng));
-    h->d_rng = nullptr;
-    HIPCHECK(hipMalloc(&h->d_rng, (size_t)nthreads * 2 * sizeof(uint64_t)));
-    h->rng_cap = nthreads;
-  }
-  HIPCHECK(hipMemsetAsync(h->d_counter, 0, 2 * sizeof(unsigned long long), h->stream));
-  HIPCHECK(hipEventRecord(h->ev[0], h->stream));
-  int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
-  if (rc) return rc;
-  HIPCHECK(hipEventRecord(h->ev[1], h->stream));
-
-  mcv::LaunchArgs A = h->args;
-  A.P = h->d_blob;
-  A.H = h->d_hist;
-  A.R = h->d_rep;
-  A.rng = h->d_rng;
-  A.counter = h->d_counter;
-  A.pool = h->pool ? h->d_counter + 1 : nullptr;
-  A.dump = h->d_dump;
-  A.dump_alive = h->d_alive;
-  A.dump_first = h->dump_first;
-  A.dump_count = h->d_dump ? h->dump_count : 0;
-  A.N = (int64_t)ncount;
-  A.nthreads = nthreads;
-  A.npt = npt;
-  A.tid_base = 0;
-  A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
-  const unsigned bs = (unsigned)h->block;
-  const int64_t per_group = (int64_t)h->block * h->spl;
-  int64_t grid = (nthreads + per_group - 1) / per_group;
-  if (h->pool) grid = persistent_grid(h->fn, h->block, A.lds_total * sizeof(double), grid);
-  void* kargs[] = {&A};
-  HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, bs, 1, 1, (unsigned)(A.lds_total * sizeof(double)),
-                                 h->stream, kargs, nullptr));
-  rc = fold_replicas(h);
-  if (rc) return rc;
-  HIPCHECK(hipEventRecord(h->ev[2], h->stream));
-  HIPCHECK(hipStreamSynchronize(h->stream));
-  unsigned long long processed = 0;
-  HIPCHECK(hipMemcpy(&processed, h->d_counter, sizeof(processed), hipMemcpyDeviceToHost));
-  float ms_rng = 0, ms_k = 0;
-  HIPCHECK(hipEventElapsedTime(&ms_rng, h->ev[0], h->ev[1]));
-  HIPCHECK(hipEventElapsedTime(&ms_k, h->ev[1], h->ev[2]));
-  if (stats) {
-    stats->processed = processed;
-    stats->ms_rng_init = ms_rng;
-    stats->ms_kernel = ms_k;
-    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  }
-  if (processed != ncount)
-    return fail(MCV_ERR_COUNT, "Processed neutron count %llu does not match requested neutron count %llu",
-                (unsigned long long)processed, (unsigned long long)ncount);
-  return MCV_OK;
+  // stream states of one launch in the handle's buffer, grown on demand
+  auto states = [h](int64_t streams, uint64_t** d_states) -> int {
+    if (streams > h->rng_cap) {
+      if (h->d_rng) HIPCHECK(hipFree(h->d_rng));
+      h->d_rng = nullptr;
+      h->rng_cap = 0;
+      HIPCHECK(hipMalloc(&h->d_rng, (size_t)streams * 2 * sizeof(uint64_t)));
+      h->rng_cap = streams;
+    }
+    *d_states = h->d_rng;
+    return MCV_OK;
+  };
+  return run_impl(h, ncount, seed, ntotalthreads, tpb, flags, buffer_size, stats, states, h->d_counter);
 }
 
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3) {
@@ -902,9 +1079,6 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
   const std::string key = std::string(code_object) + "\n" + kernel_name;
   if (key != h->adj_key) {
     if (h->adj_mod) (void)hipModuleUnload(h->adj_mod);
-  if (h->proc_mod) (void)hipModuleUnload(h->proc_mod);
-  if (h->d_io) (void)hipFree(h->d_io);
-  if (h->d_io_out) (void)hipFree(h->d_io_out);
     h->adj_mod = nullptr;
     h->adj_fn = nullptr;
     h->adj_key.clear();
@@ -917,9 +1091,15 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
       return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
     int rc = module_block(h->adj_mod, &h->adj_block, &h->adj_spl);
     if (!rc) rc = module_int(h->adj_mod, "mcv_stream_pool", 0, &h->adj_pool);
+    if (!rc) rc = module_int(h->adj_mod, "mcv_adjoint_nscore", -1, &h->adj_ns);
     if (rc) return rc;
+    kernel_info(h->adj_fn, h->adj_block, 0, &h->adj_lane_scratch, &h->adj_per_cu);
     h->adj_key = key;
   }
+  // the kernel adds into grad[0 .. NS): the caller's buffer must be exactly that long
+  if (h->adj_ns < 0) return fail(MCV_ERR_MODULE, "adjoint code object %s has no mcv_adjoint_nscore", code_object);
+  if (nscore != h->adj_ns)
+    return fail(MCV_ERR_ARG, "adjoint kernel has %d score slots, caller passed nscore=%d", h->adj_ns, nscore);
   int64_t nblocks, nthreads, npt;
   partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
   if (stats) {
@@ -957,12 +1137,14 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
   HIPCHECK(hipGetLastError());
   if (ncount > 0) {
     HIPCHECK(hipEventRecord(h->ev[0], h->stream));
-    int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
+    int rc = seed_streams(h->ctx, h->stream, seed, 0, nthreads, h->d_rng);
     if (rc) return rc;
     HIPCHECK(hipEventRecord(h->ev[1], h->stream));
     mcv::LaunchArgs A = h->args;
     A.P = h->d_blob;
     A.H = nullptr;
+    A.rng_first = 0;
+    A.tid_end = nthreads;
     A.rng = h->d_rng;
     A.counter = h->d_counter;
     A.pool = h->adj_pool ? h->d_counter + 1 : nullptr;
@@ -980,7 +1162,8 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     const unsigned bs = (unsigned)h->adj_block;
     const int64_t per_group = (int64_t)h->adj_block * h->adj_spl;
     int64_t grid = (nthreads + per_group - 1) / per_group;
-    if (h->adj_pool) grid = persistent_grid(h->adj_fn, h->adj_block, 0, grid);
+    grid = launch_grid(h->ctx, h->adj_lane_scratch, h->adj_block, h->adj_per_cu, grid, h->adj_pool != 0);
+    if (grid <= 0) return scratch_error(h->adj_lane_scratch, h->adj_block, h->ctx);
     void* kargs[] = {&A, &J};
     HIPCHECK(hipModuleLaunchKernel(h->adj_fn, (unsigned)grid, 1, 1, bs, 1, 1, 0, h->stream, kargs, nullptr));
     HIPCHECK(hipEventRecord(h->ev[2], h->stream));
@@ -1028,6 +1211,8 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
       return fail(MCV_ERR_MODULE, "kernel %s not in %s", kernel_name, code_object);
     int rc = module_block(h->proc_mod, &h->proc_block);
     if (rc) return rc;
+    kernel_info(h->proc_fn, h->proc_block, m > 0 ? 0 : h->args.lds_total * sizeof(double), &h->proc_lane_scratch,
+                &h->proc_per_cu);
     h->proc_key = key;
   }
   int64_t nblocks, nthreads, npt;
@@ -1061,7 +1246,7 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
   hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, h->stream, h->d_counter, (int64_t)1);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(h->ev[0], h->stream));
-  int rc = seed_streams(h->stream, seed, nthreads, h->d_rng);
+  int rc = seed_streams(h->ctx, h->stream, seed, 0, nthreads, h->d_rng);
   if (rc) return rc;
   HIPCHECK(hipEventRecord(h->ev[1], h->stream));
   mcv::LaunchArgs A = h->args;
@@ -1069,6 +1254,8 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
   A.H = h->d_hist;
   A.R = h->d_rep;
   A.rng = h->d_rng;
+  A.rng_first = 0;
+  A.tid_end = nthreads;
   A.counter = h->d_counter;
   A.dump = nullptr;
   A.dump_alive = nullptr;
@@ -1086,6 +1273,8 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
     A.lds_total = 0;
   }
   const unsigned bs = (unsigned)h->proc_block;
+  if (launch_grid(h->ctx, h->proc_lane_scratch, h->proc_block, h->proc_per_cu, (nthreads + bs - 1) / bs, false) <= 0)
+    return scratch_error(h->proc_lane_scratch, h->proc_block, h->ctx);
   void* kargs[] = {&A};
   HIPCHECK(hipModuleLaunchKernel(h->proc_fn, (unsigned)((nthreads + bs - 1) / bs), 1, 1, bs, 1, 1,
                                  (unsigned)(A.lds_total * sizeof(double)), h->stream, kargs, nullptr));
@@ -1123,16 +1312,19 @@ int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive) {
   return MCV_OK;
 }
 
-int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out) {
-  if (n < 0 || (n > 0 && !out)) return fail(MCV_ERR_ARG, "bad stream count");
+int mcv_rng_states(uint64_t seed, int64_t first, int64_t n, uint64_t* out) {
+  if (first < 0 || n < 0 || (n > 0 && !out)) return fail(MCV_ERR_ARG, "bad stream range");
   if (n == 0) return MCV_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MCV_ERR_HIP, "no HIP device available");
+  DeviceCtx* ctx = nullptr;
+  int rc = device_ctx(&ctx);
+  if (rc) return rc;
   uint64_t* d = nullptr;
   HIPCHECK(hipMalloc(&d, (size_t)n * 2 * sizeof(uint64_t)));
-  int rc = seed_streams(nullptr, seed, n, d);
+  rc = seed_streams(ctx, ctx->stream, seed, first, n, d);
   if (rc == MCV_OK) {
-    hipError_t e = hipDeviceSynchronize();
+    hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(MCV_ERR_HIP, "stream seeding: %s", hipGetErrorString(e));
   }
@@ -1141,6 +1333,11 @@ int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out) {
 }
 
 void mcv_instr_destroy(mcv_instr* h) { release(h); }
+
+int64_t mcv_scratch_fit(int64_t scratch_per_lane, int32_t block, int64_t resident_groups, int64_t budget_bytes) {
+  if (block < 1 || budget_bytes < 0) return 0;
+  return scratch_fit(scratch_per_lane, block, resident_groups, budget_bytes);
+}
 
 }  // extern "C"
 
@@ -1160,7 +1357,10 @@ struct mcv_batch {
   unsigned long long* d_counters = nullptr;  // processed (ninst), work item counter, stream pool
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr, fn_batch = nullptr;
+  int64_t lane_scratch = 0, lane_scratch_batch = 0;  // private segment bytes per lane
+  int per_cu = 0, per_cu_batch = 0;        // resident workgroups per CU
   hipStream_t stream = nullptr;
+  DeviceCtx* ctx = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1225,6 +1425,7 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
     int rc = device_ctx(&ctx);
     if (rc) return rc;
     b->stream = ctx->stream;  // shared, not owned
+    b->ctx = ctx;
   }
   for (auto& e : b->ev) HIPCHECK(hipEventCreate(&e));
   HIPCHECK(hipMalloc(&b->d_blob, all.size() * sizeof(double)));
@@ -1247,7 +1448,12 @@ int batch_create_impl(const mcv_comp_desc* comps, int32_t ninst, int32_t ncomp, 
   if (rc) return rc;
   if (b->item_streams < b->block * b->spl || b->item_streams % (b->block * b->spl))
     return fail(MCV_ERR_MODULE, "code object built for %d streams per work item", b->item_streams);
-  return module_int(b->mod, "mcv_stream_pool", 0, &b->pool);
+  rc = module_int(b->mod, "mcv_stream_pool", 0, &b->pool);
+  if (rc) return rc;
+  const size_t lds = b->args.lds_total * sizeof(double);
+  kernel_info(b->fn, b->block, lds, &b->lane_scratch, &b->per_cu);
+  kernel_info(b->fn_batch, b->block, lds, &b->lane_scratch_batch, &b->per_cu_batch);
+  return MCV_OK;
 }
 
 int ensure_rng(mcv_batch* b, int64_t n) {
@@ -1268,7 +1474,7 @@ int ensure_rng(mcv_batch* b, int64_t n) {
 // stream seeding + counter reset + persistent batch kernel, on b->stream
 int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t total_items, uint32_t flags,
                   int32_t grid) {
-  int rc = seed_streams(b->stream, seed, nthreads_max, b->d_rng);
+  int rc = seed_streams(b->ctx, b->stream, seed, 0, nthreads_max, b->d_rng);
   if (rc) return rc;
   hipLaunchKernelGGL(mcv_zero_u64, dim3((unsigned)((b->ninst + 1 + 255) / 256)), dim3(256), 0, b->stream,
                      b->d_counters, (int64_t)(b->ninst + 1));
@@ -1282,6 +1488,8 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.base.dump_alive = nullptr;
   B.base.dump_count = 0;
   B.base.tid_base = 0;
+  B.base.tid_end = INT64_MAX;  // each instance's own stream count bounds its streams
+  B.base.rng_first = 0;
   B.base.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
   B.pstride = b->pstride;
   B.hstride = b->hstride;
@@ -1293,7 +1501,9 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.inst_processed = b->d_counters;
   B.work = b->d_counters + b->ninst;
   B.ninst = b->ninst;
-  const int64_t g = std::min<int64_t>(grid > 0 ? grid : 1024, std::max<int64_t>(total_items, 1));
+  int64_t g = std::min<int64_t>(grid > 0 ? grid : 1024, std::max<int64_t>(total_items, 1));
+  g = launch_grid(b->ctx, b->lane_scratch_batch, b->block, b->per_cu_batch, g, true);
+  if (g <= 0) return scratch_error(b->lane_scratch_batch, b->block, b->ctx);
   void* kargs[] = {&B};
   HIPCHECK(hipModuleLaunchKernel(b->fn_batch, (unsigned)g, 1, 1, (unsigned)b->block, 1, 1,
                                  (unsigned)(B.base.lds_total * sizeof(double)), b->stream, kargs, nullptr));
@@ -1424,7 +1634,7 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     int rc = ensure_rng(b, nt);
     if (rc) return rc;
     HIPCHECK(hipEventRecord(b->ev[0], b->stream));
-    rc = seed_streams(b->stream, seed, nt, b->d_rng);
+    rc = seed_streams(b->ctx, b->stream, seed, 0, nt, b->d_rng);
     if (rc) return rc;
     hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, b->stream, b->d_counters, (int64_t)1);
     hipLaunchKernelGGL(mcv_zero_u64, dim3(1), dim3(64), 0, b->stream, b->d_counters + b->ninst + 1, (int64_t)1);
@@ -1441,11 +1651,14 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.nthreads = nt;
     A.npt = npt;
     A.tid_base = 0;
+    A.tid_end = nt;
+    A.rng_first = 0;
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
     const int64_t per_group = (int64_t)b->block * b->spl;
     int64_t grid = (nt + per_group - 1) / per_group;
-    if (b->pool) grid = persistent_grid(b->fn, b->block, A.lds_total * sizeof(double), grid);
+    grid = launch_grid(b->ctx, b->lane_scratch, b->block, b->per_cu, grid, b->pool != 0);
+    if (grid <= 0) return scratch_error(b->lane_scratch, b->block, b->ctx);
     HIPCHECK(hipModuleLaunchKernel(b->fn, (unsigned)grid, 1, 1, (unsigned)b->block, 1,
                                    1, (unsigned)(A.lds_total * sizeof(double)), b->stream, kargs,
                                    nullptr));

@@ -226,6 +226,7 @@ def process_units():
         "beamstop": lambda: single(
             __import__("acc_amd.components.optics.beamstop", fromlist=["Beamstop"]).Beamstop("b", width=1)),
         "c2-chain": configs.c2,
+        "c5-chain": configs.c5,
     }
 
 

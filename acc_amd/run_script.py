@@ -7,7 +7,10 @@ gfx950 kernel, runs ncount neutrons with the reference's stream partition and
 seeding, and saves each monitor's histogram scaled by 1/ncount
 (saveMonitorOutputs, :291-295).  Differences: the instrument is loaded once
 (the reference loads it twice, :211), `seed` is an explicit argument, and
-`buffer_size` is accepted and ignored as the reference ignores it (:316-328).
+`buffer_size`, which the reference only passes on to the instrument function
+(where it is reported as an unrecognized keyword, :316-328), splits the run
+into launches of whole streams of about that many neutrons; the streams and
+their neutrons are the single launch's, so the results do not change.
 """
 import importlib.util
 import inspect
@@ -32,12 +35,13 @@ def _check_workdir(workdir, overwrite_datafiles):
 def run(script, workdir, ncount,
         overwrite_datafiles=True,
         ntotalthreads=None, threads_per_block=None,
-        seed=None, **kwds):
+        seed=None, buffer_size=None, **kwds):
     """Run an instrument script on one GPU.
 
     script: path to the instrument script; workdir: output directory;
-    ncount: neutron count.  Extra keywords go to the script's `instrument`
-    function (unknown ones are warned about, as in the reference)."""
+    ncount: neutron count; buffer_size: neutrons per launch (None: one
+    launch).  Extra keywords go to the script's `instrument` function
+    (unknown ones are warned about, as in the reference)."""
     _check_workdir(workdir, overwrite_datafiles)
     os.makedirs(workdir)
     gate = kwds.pop("gate_dead_neutrons", config.gate_dead_neutrons)
@@ -46,7 +50,8 @@ def run(script, workdir, ncount,
     os.chdir(workdir)
     try:
         stats = run_instrument(instrument, ncount, ntotalthreads=ntotalthreads,
-                               threads_per_block=threads_per_block, seed=seed, gate_dead_neutrons=gate)
+                               threads_per_block=threads_per_block, seed=seed, gate_dead_neutrons=gate,
+                               buffer_size=buffer_size)
         saveMonitorOutputs(instrument, scale_factor=1.0 / ncount)
     finally:
         os.chdir(curdir)
@@ -87,17 +92,19 @@ def get_handle(instrument, block=None, waves_per_eu=None, streams_per_lane=None)
 
 
 def run_instrument(instrument, ncount, ntotalthreads=None, threads_per_block=None, seed=None,
-                   gate_dead_neutrons=None, accumulate=True):
+                   gate_dead_neutrons=None, accumulate=True, buffer_size=None):
     """Run ncount neutrons through an in-memory instrument; monitor `out`
     arrays receive the device histograms (added, as the reference's numba
-    round trip of `out` accumulates)."""
+    round trip of `out` accumulates).  buffer_size: neutrons per launch
+    (whole streams; None or 0: one launch)."""
     from .components.monitors.MonitorBase import MonitorBase
     h = get_handle(instrument)
     seed = config.rng_seed if seed is None else seed
     gate = config.gate_dead_neutrons if gate_dead_neutrons is None else gate_dead_neutrons
     h.clear()
     stats = h.run(int(ncount), seed, ntotalthreads or config.ntotalthreads,
-                  threads_per_block or config.threads_per_block, MCV_FLAG_GATE_DEAD if gate else 0)
+                  threads_per_block or config.threads_per_block, MCV_FLAG_GATE_DEAD if gate else 0,
+                  buffer_size=int(buffer_size or 0))
     for i, c in enumerate(instrument.components):
         if isinstance(c, MonitorBase):
             out = h.read_monitor(i).reshape(c.out.shape)
@@ -159,11 +166,13 @@ def main(argv=None):
     ap.add_argument("--total_threads", type=int, default=int(1e6))
     ap.add_argument("--threads_per_block", type=int, default=512)
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--buffer_size", type=float, default=None)
     ap.add_argument("--additional-kargs", dest="additional_kargs", default=None)
     a = ap.parse_args(argv)
     kwds = yaml.safe_load(open(a.additional_kargs)) if a.additional_kargs else {}
     stats = run(a.script, a.workdir, int(a.ncount), overwrite_datafiles=a.overwrite_datafiles,
-                ntotalthreads=a.total_threads, threads_per_block=a.threads_per_block, seed=a.seed, **kwds)
+                ntotalthreads=a.total_threads, threads_per_block=a.threads_per_block, seed=a.seed,
+                buffer_size=int(a.buffer_size) if a.buffer_size else None, **kwds)
     print(stats)
     return 0
 

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MCV_ABI_VERSION 1
+#define MCV_ABI_VERSION 2
 
 /* ---- error classes (mirrors the exception types the reference raises) ---- */
 #define MCV_OK 0
@@ -33,7 +33,7 @@ extern "C" {
 #define MCV_ERR_HIP -2     /* HIP runtime failure (no device, launch failure) */
 #define MCV_ERR_MODULE -3  /* code object missing or kernel symbol absent */
 #define MCV_ERR_COUNT -4   /* processed-neutron count mismatch (run_script.py:218-222) */
-#define MCV_ERR_NOMEM -5
+#define MCV_ERR_NOMEM -5   /* a launch's lane scratch exceeds the budget (MCV_SCRATCH_BUDGET_MB) */
 
 /* ---- component type tags ------------------------------------------------
  * params/iparams/data layouts (float64 unless stated):
@@ -134,6 +134,8 @@ typedef struct mcv_run_stats {
   double ms_rng_init;     /* device time of the stream-seeding kernel */
   double ms_kernel;       /* device time of the fused propagate kernel(s) */
   double ms_total;        /* host wall time of the call */
+  int64_t nchunks;        /* launches the run was split into (buffer_size, scratch budget) */
+  int64_t scratch_per_lane; /* private segment bytes per lane of the launched kernel */
 } mcv_run_stats;
 
 typedef struct mcv_instr mcv_instr;
@@ -168,10 +170,16 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params,
  * and numba's xoroshiro128+ seeding are reproduced, so stream tid runs neutrons
  * [tid*npt, min((tid+1)*npt, ncount)).  Histograms accumulate across calls
  * until mcv_monitor_clear.  ntotalthreads/tpb <= 0 select 1e6 / 512
- * (acc/config.py:1-2).  flags: MCV_FLAG_*. */
+ * (acc/config.py:1-2).  flags: MCV_FLAG_*.
+ * buffer_size (run_script.run's keyword, which the reference accepts and
+ * ignores) > 0 splits the run into launches of whole streams holding about
+ * buffer_size neutrons (at least one stream): launch c runs streams
+ * [c*S, (c+1)*S) of the same partition, seeded as in one launch, so every
+ * neutron is computed exactly as in a single launch.  The runtime also splits
+ * a launch whose lanes' scratch would exceed the budget. */
 int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed,
                   int64_t ntotalthreads, int32_t tpb, uint32_t flags,
-                  mcv_run_stats* stats);
+                  uint64_t buffer_size, mcv_run_stats* stats);
 
 /* Copy the (3, n_bins) histogram of monitor component `comp` into out3. */
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3);
@@ -216,10 +224,19 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
                       uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags, const double* g_hist,
                       int32_t nscore, double* grad, int64_t ndense, double* grad_dense, mcv_run_stats* stats);
 
-/* Seed n xoroshiro128+ streams exactly as numba's create_xoroshiro128p_states
- * (SplitMix64(seed) into both words, stream i = stream i-1 jumped 2^64) and copy
- * the (n, 2) uint64 states to host memory.  Used to pin the device seeding. */
-int mcv_rng_states(uint64_t seed, int64_t n, uint64_t* out);
+/* Seed streams [first, first + n) exactly as numba's create_xoroshiro128p_states
+ * does (SplitMix64(seed) into both words, stream i = stream i-1 jumped 2^64) and
+ * copy the (n, 2) uint64 states to host memory.  Used to pin the device seeding. */
+int mcv_rng_states(uint64_t seed, int64_t first, int64_t n, uint64_t* out);
+
+/* Scratch guard arithmetic (no device work): how many workgroups of `block`
+ * lanes needing scratch_per_lane bytes each may be resident together within
+ * budget_bytes, at most resident_groups; 0 when not even one fits.  Every
+ * launch applies it with the kernel's private segment size and the device's
+ * budget (MCV_SCRATCH_BUDGET_MB, default an eighth of the free device memory):
+ * persistent grids are narrowed, mcv_instr_run splits its launch, and the other
+ * entry points return MCV_ERR_NOMEM.  No reference counterpart. */
+int64_t mcv_scratch_fit(int64_t scratch_per_lane, int32_t block, int64_t resident_groups, int64_t budget_bytes);
 
 /* Release the handle and its device memory. */
 void mcv_instr_destroy(mcv_instr* h);

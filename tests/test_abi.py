@@ -33,7 +33,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_abi_version_and_constants():
     L = _abi.lib()
-    assert L.mcv_abi_version() == 1
+    assert L.mcv_abi_version() == _abi.ABI_VERSION == 2
     text = open(os.path.join(ROOT, "include", "mcv_acc.h")).read()
     for name in ("MCV_SOURCE_SIMPLE", "MCV_GUIDE", "MCV_GUIDE_TAPERING", "MCV_SAMPLE_HSS", "MCV_SAMPLE_HMS",
                  "MCV_MON_PSD", "MCV_MON_WAVELENGTH", "MCV_MON_IQE", "MCV_DET_TUBES", "MCV_SRC_IQE_NORMALIZER",
@@ -56,3 +56,36 @@ def test_no_device_errors_cleanly():
         _abi.Handle([c.mcv_desc() for c in instr.components], off, rot, "/nonexistent.hsaco", "k")
     with pytest.raises(_abi.McvError):
         _abi.rng_states(1, 10)
+
+
+def test_scratch_fit_arithmetic():
+    """Workgroups whose lanes' scratch fits a budget: budget // (bytes x lanes),
+    at most the resident count, 0 when one does not fit, unbounded without
+    scratch."""
+    fit = _abi.scratch_fit
+    assert fit(0, 256, 512, 0) == 512
+    assert fit(25632, 64, 10 ** 6, 1 << 30) == (1 << 30) // (25632 * 64)
+    assert fit(25632, 64, 100, 1 << 30) == 100
+    assert fit(25632, 64, 100, 25632 * 64 - 1) == 0
+    assert fit(25632, 64, 100, 25632 * 64) == 1
+    assert fit(100, 128, 0, 1 << 30) == 0
+    assert fit(100, 0, 5, 1 << 30) == 0
+
+
+def test_scratch_guard_on_ms_kernel_metadata():
+    """The multiple-scattering kernel with the reference's 100-neutron lists
+    keeps tens of KB per lane in scratch (code-object metadata of the built
+    kernel): at the device's full residency (256 CUs, 2 one-wave workgroups
+    per SIMD) that is several GB, so a small budget narrows the grid and the
+    forward kernel runs in more launches."""
+    from acc_amd import prebuilt
+    from acc_amd.fuser import fuse
+    from codeobj_tools import private_segment_sizes
+    path, name = fuse(prebuilt.instruments()["hms-ref"]())
+    sizes = private_segment_sizes(path)
+    per_lane = sizes[name]
+    assert sizes[name + "_batch"] > 0 and per_lane > 16384, sizes
+    resident = 256 * 4 * 2
+    full = per_lane * 64 * resident
+    assert _abi.scratch_fit(per_lane, 64, resident, full) == resident
+    assert _abi.scratch_fit(per_lane, 64, resident, full // 8) == resident // 8

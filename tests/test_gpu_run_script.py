@@ -7,6 +7,7 @@ import sys
 import numpy as np
 import pytest
 
+import pyoracle
 from acc_amd import run_script
 from acc_amd.histogram import load
 
@@ -67,3 +68,76 @@ def test_cli(script, tmp_path):
                        cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     assert r.returncode == 0, r.stderr[-2000:]
     assert os.path.exists(os.path.join(workdir, "psd.npz"))
+
+
+C1_SCRIPT = """
+from acc_amd import configs
+
+def instrument():
+    return configs.c1()
+"""
+
+
+def test_saved_histograms_are_the_scaled_sums(tmp_path, monkeypatch):
+    """What run() saves is the reference's finalization of the raw sums
+    (acc/run_script.py:339-343 saveMonitorOutputs with scale_factor 1/ncount;
+    psd_monitor.py:53-59 getHistogram: I = sum p * s, E2 = sum p^2 * s^2;
+    iqe_monitor.py:57-60 getHistogram: that histogram divided by the
+    Monte-Carlo normalizer, errors propagated as for a quotient), checked
+    against the CPU oracle's raw sums for the same neutrons; a run split into
+    launches by buffer_size saves the same counts."""
+    from acc_amd.components.monitors.MonitorBase import MonitorBase
+    from acc_amd.components.monitors.iqe_monitor import IQE_monitor, getNormalization
+    from acc_amd.instrument import calcTransformations
+    p = tmp_path / "c1_instrument.py"
+    p.write_text(C1_SCRIPT)
+    ncount, seed, ntot = 300000, 3, 20000
+    seen = []
+    orig = run_script.saveMonitorOutputs
+
+    def spy(instrument, scale_factor=1.0):
+        seen.append((instrument, scale_factor))
+        return orig(instrument, scale_factor=scale_factor)
+
+    monkeypatch.setattr(run_script, "saveMonitorOutputs", spy)
+    saved, instrs = {}, {}
+    for label, buf in (("one", None), ("chunked", 37000)):
+        workdir = str(tmp_path / label)
+        st = run_script.run(str(p), workdir, ncount, ntotalthreads=ntot, seed=seed, buffer_size=buf)
+        streams = -(-ncount // st["npt"])
+        assert st["processed"] == ncount
+        assert st["nchunks"] == (1 if buf is None else -(-streams // (buf // st["npt"])))
+        instr, sf = seen[-1]
+        assert sf == 1.0 / ncount
+        mons = [(i, c) for i, c in enumerate(instr.components) if isinstance(c, MonitorBase)]
+        saved[label] = {c.name: load(os.path.join(workdir, os.path.splitext(c.filename)[0] + ".npz"))
+                        for _, c in mons}
+        instrs[label] = instr
+    instr = instrs["one"]
+    mons = [(i, c) for i, c in enumerate(instr.components) if isinstance(c, MonitorBase)]
+    descs = [c.mcv_desc() for c in instr.components]
+    off, rot = calcTransformations(instr)
+    hists, processed, _, _ = pyoracle.run(descs, off, rot, ncount, seed, ntotalthreads=ntot, tpb=512, gate=True)
+    assert processed == ncount
+    for i, c in mons:
+        raw = hists[i].reshape(c.out.shape)
+        assert raw[0].sum() > 100, c.name
+        # the monitor's raw sums are the oracle's (counts exact)
+        np.testing.assert_array_equal(c.out[0], raw[0])
+        np.testing.assert_allclose(c.out[1:], raw[1:], rtol=1e-9, atol=1e-300)
+        h = saved["one"][c.name]
+        np.testing.assert_array_equal(instrs["chunked"].components[i].out[0], c.out[0])
+        np.testing.assert_array_equal(saved["chunked"][c.name].I > 0, h.I > 0)
+        if isinstance(c, IQE_monitor):
+            n = getNormalization(c)
+            a, ea = raw[1] / ncount, raw[2] / ncount ** 2
+            exp_I = a / n.I
+            exp_E2 = ea / n.I ** 2 + a ** 2 * n.E2 / n.I ** 4
+            assert (n.I > 0).all() and (h.I != raw[1]).any()
+        else:
+            exp_I, exp_E2 = raw[1] / ncount, raw[2] / ncount ** 2
+            # exactly the monitor's sums times 1/ncount
+            np.testing.assert_array_equal(h.I, c.out[1] * (1.0 / ncount))
+            np.testing.assert_array_equal(h.E2, c.out[2] * (1.0 / ncount) * (1.0 / ncount))
+        np.testing.assert_allclose(h.I, exp_I, rtol=1e-9, atol=1e-300, err_msg=c.name)
+        np.testing.assert_allclose(h.E2, exp_E2, rtol=1e-9, atol=1e-300, err_msg=c.name)
