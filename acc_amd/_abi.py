@@ -57,6 +57,7 @@ EXPORTED = (
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
     "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process", "mcv_scratch_fit",
+    "mcv_instr_run_noalloc",
 )
 ABI_VERSION = 2
 
@@ -123,6 +124,8 @@ def lib():
     L.mcv_instr_run.restype = ctypes.c_int
     L.mcv_instr_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
                                 ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(McvRunStats)]
+    L.mcv_instr_run_noalloc.restype = ctypes.c_int
+    L.mcv_instr_run_noalloc.argtypes = L.mcv_instr_run.argtypes
     L.mcv_monitor_read.restype = ctypes.c_int
     L.mcv_monitor_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, dp]
     L.mcv_monitor_clear.restype = ctypes.c_int
@@ -232,6 +235,15 @@ class Handle:
         st = McvRunStats()
         check(lib().mcv_instr_run(self.h, int(ncount), int(seed), int(ntotalthreads or 0),
                                   int(threads_per_block or 0), int(flags), int(buffer_size or 0), ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_noalloc(self, ncount, seed, ntotalthreads=0, threads_per_block=0, flags=MCV_FLAG_GATE_DEAD,
+                    buffer_size=0):
+        """mcv_instr_run_noalloc: as run(), with no device allocation."""
+        st = McvRunStats()
+        check(lib().mcv_instr_run_noalloc(self.h, int(ncount), int(seed), int(ntotalthreads or 0),
+                                          int(threads_per_block or 0), int(flags), int(buffer_size or 0),
+                                          ctypes.byref(st)))
         return st.as_dict()
 
     def set_params(self, comp, params):

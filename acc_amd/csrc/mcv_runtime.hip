@@ -175,6 +175,16 @@ __global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restri
   }
 }
 
+// Scratch of mcv_instr_run_noalloc: the launch counters (processed neutrons,
+// stream pool) in the first kScratchCounterWords words, then the states of up
+// to kScratchStreams streams (two words each) -- 16 MiB + 64 B per device,
+// held from the library's first use on a device until the process ends.  In
+// host code `scratch` names a host-side stand-in; the device address is
+// DeviceCtx::scratch, from hipGetSymbolAddress at the device's set-up.
+constexpr int64_t kScratchCounterWords = 8;
+constexpr int64_t kScratchStreams = int64_t(1) << 20;
+static __device__ __align__(256) uint64_t scratch[kScratchCounterWords + 2 * kScratchStreams];
+
 // Per-device state shared by every handle: the jump tables and one stream.
 // All handles of a device queue their work on that one stream (calls are
 // synchronous anyway): the runtime reserves a kernel's scratch (private
@@ -186,6 +196,7 @@ struct DeviceCtx {
   hipStream_t stream = nullptr;
   int cus = 1;                 // compute units
   int64_t scratch_budget = 0;  // bytes of lane scratch a launch may keep resident
+  uint64_t* scratch = nullptr; // this device's copy of `scratch`
 };
 
 // Budget for the scratch (private segment) that the lanes of one launch keep
@@ -225,6 +236,9 @@ int device_ctx(DeviceCtx** out) {
   HIPCHECK(hipStreamCreate(&c.stream));
   HIPCHECK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
   c.scratch_budget = default_scratch_budget();
+  // the no-allocation entry point's buffer: its address on this device (current here)
+  HIPCHECK(hipGetSymbolAddress((void**)&c.scratch, HIP_SYMBOL(scratch)));
+  if (!c.scratch) return fail(MCV_ERR_HIP, "no device address for the no-allocation scratch buffer");
   g_ctx.push_back(c);
   *out = &g_ctx.back();
   return MCV_OK;
@@ -1029,6 +1043,24 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalth
     return MCV_OK;
   };
   return run_impl(h, ncount, seed, ntotalthreads, tpb, flags, buffer_size, stats, states, h->d_counter);
+}
+
+int mcv_instr_run_noalloc(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb,
+                          uint32_t flags, uint64_t buffer_size, mcv_run_stats* stats) {
+  if (!h) return fail(MCV_ERR_ARG, "null handle");
+  // stream states and counters in the device's static scratch: nothing is allocated
+  uint64_t* base = h->ctx->scratch;
+  auto states = [base](int64_t streams, uint64_t** d_states) -> int {
+    if (streams > kScratchStreams)
+      return fail(MCV_ERR_ARG,
+                  "mcv_instr_run_noalloc: a launch of %lld streams exceeds the scratch buffer's %lld streams; "
+                  "split the run with a buffer_size of at most %lld streams' worth of neutrons",
+                  (long long)streams, (long long)kScratchStreams, (long long)kScratchStreams);
+    *d_states = base + kScratchCounterWords;
+    return MCV_OK;
+  };
+  return run_impl(h, ncount, seed, ntotalthreads, tpb, flags, buffer_size, stats, states,
+                  reinterpret_cast<unsigned long long*>(base));
 }
 
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3) {

@@ -181,6 +181,21 @@ int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed,
                   int64_t ntotalthreads, int32_t tpb, uint32_t flags,
                   uint64_t buffer_size, mcv_run_stats* stats);
 
+/* mcv_instr_run without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No reference counterpart.  Same arguments, partition,
+ * streams and results as mcv_instr_run; the stream states and launch
+ * counters live in a static per-device buffer of the library (16 MiB + 64 B,
+ * held until the process ends) instead of the handle's allocations, so a
+ * launch may hold at most 2^20 streams (the default partition holds
+ * 1,000,448): a larger one is refused with MCV_ERR_ARG unless buffer_size
+ * splits it.  Counters are cleared by the call itself.  Every call on one GPU
+ * shares that buffer: calls that overlap on one GPU (from several host
+ * threads) race. */
+int mcv_instr_run_noalloc(mcv_instr* h, uint64_t ncount, uint64_t seed,
+                          int64_t ntotalthreads, int32_t tpb, uint32_t flags,
+                          uint64_t buffer_size, mcv_run_stats* stats);
+
 /* Copy the (3, n_bins) histogram of monitor component `comp` into out3. */
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3);
 
