@@ -22,8 +22,13 @@
 //    (mcni.utils.conversion is not vendored); K2V = 1/V2K -- parity unpinned.
 //  * Geometry, HSS time bookkeeping, HMS interactions, frame transforms: pinned
 //    by the reference tests' literal vectors (tests/golden/*.json).
-//  * S(Q,E)-grid kernel, phonon-DOS kernel, tube detector: no reference
-//    counterpart -- parity unpinned; checked by closed forms only.
+//  * Source_simple (draw order, float32 radius, mirrored target point), the
+//    S(Q,E)-grid lookup, the phonon-DOS weight, the tube detector, wavelength
+//    binning, the I(Q,E) normalizer's neutrons and the segmented guide's
+//    sloped-wall reflection: pinned by closed forms computed from the
+//    neutron's own state (tests/test_oracle_pins.py); the grid and DOS kernels
+//    and the detector have no reference counterpart, so those pins state what
+//    this build defines them to be.
 
 #include <cmath>
 #include <cstdint>
@@ -1729,6 +1734,7 @@ int orc_component(const mcv_comp_desc* d, double* n, uint64_t* state2, double* h
   int ret = 0;
   switch (d->type) {
     case MCV_SOURCE_SIMPLE: source_simple(rng, n, (const double*)d->params); break;
+    case MCV_SRC_IQE_NORMALIZER: iqe_normalizer_source(rng, n, (const double*)d->params); break;
     case MCV_GUIDE: guide(n, (const double*)d->params); break;
     case MCV_SLIT: aperture(n, (const double*)d->params, false); break;
     case MCV_BEAMSTOP: aperture(n, (const double*)d->params, true); break;
@@ -1754,7 +1760,7 @@ int orc_component(const mcv_comp_desc* d, double* n, uint64_t* state2, double* h
     case MCV_MON_WAVELENGTH: wavelength_monitor(n, (const double*)d->params, d->iparams[0], hist); break;
     case MCV_MON_IQE: iqe_monitor(n, (const double*)d->params, d->iparams[0], d->iparams[1], hist); break;
     case MCV_DET_TUBES: tube_detector(n, (const double*)d->params, d->iparams[0], d->iparams[1], d->iparams[2], hist); break;
-    default: break;
+    default: return -1;  // not a component this entry point runs
   }
   state2[0] = rng.s0;
   state2[1] = rng.s1;

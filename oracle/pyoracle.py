@@ -184,6 +184,8 @@ def component(desc, neutron, state=(0, 0), which=0):
         M = int(desc["iparams"][2])
     out = np.zeros((max(M, 2), 10))
     ret = lib().orc_component(D.arr, _d(n), st.ctypes.data_as(_up), _d(hist), _d(out), int(which))
+    if ret < 0:
+        raise ValueError("orc_component does not run component type %d" % desc["type"])
     return n, st, hist, out[:ret] if desc["type"] == 5 else None
 
 
