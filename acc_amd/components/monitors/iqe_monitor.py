@@ -1,8 +1,8 @@
-"""IQE_monitor (acc/components/monitors/iqe_monitor.py:28-344).
+"""IQE_monitor (acc/components/monitors/iqe_monitor.py:15-174).
 
 Bins (Q, E) of the scattered neutron inside an angular window; position is not
 used.  `getHistogram` divides by a Monte-Carlo normalizer (getNormalization,
-:286-344) computed on the device with a fixed seed, where the reference uses
+:115-174) computed on the device with a fixed seed, where the reference uses
 unseeded numpy draws.  Device routine: mcv::iqe_bin.
 """
 import numpy as np
@@ -90,7 +90,7 @@ class IQE_monitor(base):
 
 
 def getNormalization(monitor, N=None, epsilon=1e-7, seed=0):
-    """Monte-Carlo normalizer of the IQE histogram (iqe_monitor.py:286-315):
+    """Monte-Carlo normalizer of the IQE histogram (iqe_monitor.py:115-144):
     N = nQ*nE*1e4 neutrons with E uniform in [Emin, Emax], isotropic final
     directions, speed e2v(Ei - E) and weight vf/vi, binned by a copy of the
     monitor; bins below epsilon are set to 1."""

@@ -1,5 +1,5 @@
 """PosDiv_monitor: intensity vs x and horizontal divergence at z = 0
-(acc/components/monitors/posdiv_monitor.py:14-95).
+(acc/components/monitors/posdiv_monitor.py:16-89).
 
 Non-destructive: propagates to z = 0 without writing back, keeps neutrons
 strictly inside the window and the divergence range, bins

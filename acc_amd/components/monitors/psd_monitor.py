@@ -1,4 +1,4 @@
-"""PSD_monitor (acc/components/monitors/psd_monitor.py:29-96).
+"""PSD_monitor (acc/components/monitors/psd_monitor.py:16-83).
 
 Propagates the neutron to z=0 without writing it back, bins (x, y) strictly
 inside the window, adds N, p, p^2.  Device routine: mcv::psd_bin.

@@ -1,5 +1,5 @@
 """PSD_monitor_4Pi: intensity on a sphere around the origin vs (phi, theta)
-(acc/components/monitors/psd_monitor_4pi.py:14-81).
+(acc/components/monitors/psd_monitor_4pi.py:17-79).
 
 Destructive as the reference's: the neutron is moved onto the sphere (its
 first crossing ahead), phi = atan2(x, z), theta = asin(y / radius), with the

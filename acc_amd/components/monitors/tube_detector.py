@@ -5,7 +5,7 @@ component origin; tube j covers the azimuth phi = atan2(x, z) in
 [phi_min + j dphi, phi_min + (j+1) dphi) (degrees), each split into `npixels`
 along y over `height`, and each pixel histograms time of flight in `ntof`
 bins over [tof_min, tof_max) seconds.  Like the reference monitors
-(acc/components/monitors/psd_monitor.py:79-96) it does not change the neutron
+(acc/components/monitors/psd_monitor.py:66-83) it does not change the neutron
 and accumulates N, sum p, sum p^2 -- here with FP64 atomics straight into HBM,
 since the (3, ntubes*npixels*ntof) histogram is larger than any cache.
 Device routine: mcv::tube_bin.

@@ -1,4 +1,4 @@
-"""Wavelength_monitor (acc/components/monitors/wavelength_monitor.py:15-171).
+"""Wavelength_monitor (acc/components/monitors/wavelength_monitor.py:15-74).
 
 Propagates to z=0 without writing back, lambda = 2 pi / (v V2K), adds N, p,
 p^2.  Device routine: mcv::wavelength_bin.

@@ -1,4 +1,4 @@
-"""Beamstop at z = 0, rectangular or circular (acc/components/optics/beamstop.py:17-103).
+"""Beamstop at z = 0, rectangular or circular (acc/components/optics/beamstop.py:17-112).
 
 The complement of the slit: neutrons inside the stop (or below the weight
 cut) are absorbed (p = -1); the others move to z = 0; a neutron that never

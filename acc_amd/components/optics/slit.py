@@ -1,4 +1,4 @@
-"""Slit at z = 0, rectangular or circular (acc/components/optics/slit.py:17-105).
+"""Slit at z = 0, rectangular or circular (acc/components/optics/slit.py:17-108).
 
 A neutron that never reaches z = 0 passes unchanged; one below the weight cut
 or outside the opening is absorbed (p = -1); the others move to z = 0.
@@ -13,7 +13,7 @@ category = 'optics'
 
 
 def aperture_params(kind, xmin, xmax, ymin, ymax, radius, cut, width, height):
-    """The reference's argument handling for Slit and Beamstop (slit.py:43-60)."""
+    """The reference's argument handling for Slit and Beamstop (slit.py:45-60)."""
     if width > 0:
         xmax = width / 2
         xmin = -xmax

@@ -6,7 +6,7 @@ max_ms_loops_path1, minimum_neutron_event_probability)` returns a sample
 class whose propagate emits up to max_scattered_neutrons output neutrons per
 incident neutron (transmitted and scattered branches, bounded breadth-first
 loops); the fuser runs every downstream component on each output
-(acc/run_script.py:225-230).
+(the loop acc/run_script.py:87-92 generates).
 """
 from ..._abi import MCV_SAMPLE_HMS
 from ..ComponentBase import DeviceFunction

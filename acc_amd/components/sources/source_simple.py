@@ -46,7 +46,7 @@ class Source_simple(SourceBase):
         if (E0 == 0 and dE == 0 and (Lambda0 <= 0 or dLambda < 0 or Lambda0 - dLambda <= 0)):
             raise RuntimeError("bad wavelength distribution spec")
         if gauss:
-            # accepted and ignored, as in the reference (source_simple.py:154-215)
+            # accepted and ignored, as in the reference (source_simple.py:82-83, 104-108)
             warnings.warn("Source_simple: gauss is ignored; the distribution is flat")
         wl_distr = Lambda0 != 0
         self.propagate_params = (

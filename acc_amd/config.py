@@ -4,9 +4,9 @@
 (acc/components/sources/SourceBase.py:18), and `run()` also takes an explicit
 `seed`.  `gate_dead_neutrons` selects the histogram semantics: True drops a
 neutron once its weight p <= 0 (the mcvine engine's, and the reference's
-buffered mode, acc/components/ComponentBase.py:219-221); False reproduces the
+buffered mode, acc/components/ComponentBase.py:100-102); False reproduces the
 reference fused kernel, where an absorbed neutron keeps flowing downstream
-(acc/run_script.py:182-196, acc/neutron.py:48-58).  `kernel_block` is the
+(acc/run_script.py:182-196, acc/neutron.py:25-35).  `kernel_block` is the
 GPU workgroup size the fuser compiles for.
 """
 ntotalthreads = int(1e6)

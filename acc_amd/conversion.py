@@ -1,4 +1,4 @@
-"""Neutron unit conversions (mcni.utils.conversion, used by acc/neutron.py:92-106).
+"""Neutron unit conversions (mcni.utils.conversion, used by acc/neutron.py:69-83).
 
 mcni is not vendored with the reference; the values are McStas' published
 constants.  K2V is taken as 1/V2K.

@@ -82,7 +82,7 @@ __device__ __forceinline__ double calc_reflectivity(double Q, double R0, double 
 // ------------------------------------------------------------------ exit-time selection
 // The guides pick the earliest of several exit times t_k = n_k / d_k with a
 // chain of strict "t2 < t1" tests over the walls the neutron moves towards
-// (d_k < 0; guide.py:100-123, guide_tapering.py:206-230).  Here the chain
+// (d_k < 0; guide.py:100-123, guide_tapering.py:204-227).  Here the chain
 // compares quotients by cross-multiplication, without branches: a losing
 // candidate costs two products and a few selects instead of an FP64 division
 // (the costliest instruction sequence of the loop, which a wavefront would
@@ -221,7 +221,7 @@ __device__ __forceinline__ void guide_bounces(Neutron& n, const double* __restri
 }
 
 // ------------------------------------------------------------------ Slit / Beamstop
-// acc/components/optics/slit.py:72-105, beamstop.py:70-103.
+// acc/components/optics/slit.py:71-108, beamstop.py:75-112.
 // P = xmin, xmax, ymin, ymax, radius^2, cut.  A neutron that never reaches
 // z = 0 is left as it is; below the weight cut or stopped it is absorbed
 // (p = -1, the rest of the state unchanged); otherwise it moves to z = 0.
@@ -321,7 +321,7 @@ __device__ __forceinline__ void taper_init(TaperState& s, const Neutron& n, cons
   s.died = false;
 }
 
-// Applies the pending reflection (guide_tapering.py:240-284); returns true
+// Applies the pending reflection (guide_tapering.py:236-284); returns true
 // when it kills the neutron.
 template <class T>
 __device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __restrict__ P,
@@ -355,7 +355,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
                                            const double* __restrict__ D, T& tr) {
   const double l_seg = P[0];
   const int nseg = (int)P[9];
-  if (s.enter) {  // segment entrance: guide_tapering.py:170-190
+  if (s.enter) {  // segment entrance: guide_tapering.py:170-186
     const double* __restrict__ rec = D + 16 * s.seg;
     const double w1s = rec[0], hhs = rec[1];
     s.wws = rec[2];
@@ -386,7 +386,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
   } else if (s.seg >= nseg) {
     return true;
   }
-  // one pass of the bounce loop (guide_tapering.py:193-284); nb counts the
+  // one pass of the bounce loop (guide_tapering.py:189-284); nb counts the
   // reference's max_bounces passes per segment
   bool seg_end = s.nb >= 100000;
   const double zr = s.zr, wws = s.wws, hhh = s.hhh, z = s.z, x = s.x, y = s.y, vx = s.vx, vy = s.vy, vz = s.vz;
@@ -445,7 +445,7 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
 }
 
 // A dead neutron keeps the state it entered with (p = 0); a transmitted one
-// takes the guide's final state (guide_tapering.py:286-289).
+// takes the guide's final state (guide_tapering.py:285-289).
 __device__ __forceinline__ void taper_finish(const TaperState& s, Neutron& n) {
   if (s.died) {
     n.p = 0;
@@ -475,7 +475,7 @@ __device__ __forceinline__ void guide_tapering(Neutron& n, const double* __restr
 // Each monitor is its binning (`*_bin`: the bin index, or -1 when the neutron
 // is not counted, and the number of bins); the fused kernel adds (1, p, p^2)
 // to the bin (mcv::hist_count) and the adjoint kernel weighs the same bin.
-// acc/neutron.py:83-89 prop_z0 (monitors do not write it back)
+// acc/neutron.py:60-66 prop_z0 (monitors do not write it back)
 __device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, double& t) {
   const double dt = -n.z / n.vz;
   x = n.x + n.vx * dt;
@@ -483,7 +483,7 @@ __device__ __forceinline__ void prop_z0(const Neutron& n, double& x, double& y, 
   t = n.t + dt;
 }
 
-// acc/components/monitors/psd_monitor.py:79-96.  P = xmin, xmax, ymin, ymax, nx, ny
+// acc/components/monitors/psd_monitor.py:66-83.  P = xmin, xmax, ymin, ymax, nx, ny
 __device__ __forceinline__ int64_t psd_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
   const int64_t nx = (int64_t)P[4], ny = (int64_t)P[5];
   nbins = nx * ny;
@@ -500,7 +500,7 @@ __device__ __forceinline__ int64_t psd_bin(const Neutron& n, const double* __res
   return ix * ny + iy;
 }
 
-// acc/components/monitors/wavelength_monitor.py:149-171.  P = xmin..ymax, Lmin, Lmax, nchan
+// acc/components/monitors/wavelength_monitor.py:52-74.  P = xmin..ymax, Lmin, Lmax, nchan
 __device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double* __restrict__ P,
                                                   int64_t& nbins) {
   const int64_t nchan = (int64_t)P[6];
@@ -518,7 +518,7 @@ __device__ __forceinline__ int64_t wavelength_bin(const Neutron& n, const double
   return iL;
 }
 
-// acc/components/monitors/posdiv_monitor.py:70-95.  P = xmin, xmax, ymin, ymax, maxdiv, npos, ndiv
+// acc/components/monitors/posdiv_monitor.py:67-89.  P = xmin, xmax, ymin, ymax, maxdiv, npos, ndiv
 __device__ __forceinline__ int64_t posdiv_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
   const int64_t npos = (int64_t)P[5], ndiv = (int64_t)P[6];
   nbins = npos * ndiv;
@@ -536,7 +536,7 @@ __device__ __forceinline__ int64_t posdiv_bin(const Neutron& n, const double* __
   return ix * ndiv + idiv;
 }
 
-// acc/components/monitors/psd_monitor_4pi.py:59-81.  P = phimin, phimax,
+// acc/components/monitors/psd_monitor_4pi.py:56-79.  P = phimin, phimax,
 // thetamin, thetamax, radius, nphi, ntheta.  Moves the neutron onto the sphere
 // (the reference's monitor is destructive) and bins (phi, theta) with the
 // reference's index formulas (theta's rounds half up).  A neutron whose line
@@ -568,7 +568,7 @@ __device__ __forceinline__ int64_t psd4pi_bin(Neutron& n, const double* __restri
   return iphi * ntheta + itheta;
 }
 
-// acc/components/monitors/iqe_monitor.py:251-284.
+// acc/components/monitors/iqe_monitor.py:80-113.
 // P = Ei, Qmin, Qmax, Emin, Emax, max_in, min_in, max_out, min_out, nQ, nE
 __device__ __forceinline__ int64_t iqe_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
   const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
@@ -641,7 +641,7 @@ __device__ __forceinline__ void storage_source(Neutron& n, const double* __restr
 
 // ------------------------------------------------------------------ IQE normalizer source
 // Neutrons of the I(Q,E) monitor's Monte-Carlo normalizer
-// (acc/components/monitors/iqe_monitor.py:317-344, makeNeutrons): energy
+// (acc/components/monitors/iqe_monitor.py:146-174, makeNeutrons): energy
 // transfer E uniform in [Emin, Emax), final speed e2v(Ei - E), direction
 // uniform on 4 pi, weight vf/vi, at the origin.  P = Ei, Emin, Emax.
 __device__ __forceinline__ void iqe_normalizer_source(Rng& rng, Neutron& n, const double* __restrict__ P) {

@@ -43,7 +43,7 @@ struct Rng {
 };
 
 // ------------------------------------------------------------------ frames
-// acc/neutron.py:41-46 (abs2rel): r' = R (r - o); v' = R v.  T = {R[9], o[3]}.
+// acc/neutron.py:18-22 (abs2rel): r' = R (r - o); v' = R v.  T = {R[9], o[3]}.
 __device__ __forceinline__ void apply_transformation(Neutron& n, const double* __restrict__ T) {
   const double rx = n.x - T[9], ry = n.y - T[10], rz = n.z - T[11];
   const double vx = n.vx, vy = n.vy, vz = n.vz;

@@ -184,7 +184,7 @@ __device__ __forceinline__ double tof_before_first_exit(const double* S, const N
   return t;
 }
 
-__device__ __forceinline__ bool is_moving(const Neutron& n) {  // acc/neutron.py:60-65
+__device__ __forceinline__ bool is_moving(const Neutron& n) {  // acc/neutron.py:38-42
   return sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz) > 1.e-6;
 }
 
@@ -241,7 +241,7 @@ __device__ __forceinline__ int interactM1(Rng& rng, const double* P, const doubl
   prop_dt_inplace(out[0], tof);
   out[0].p *= transmission_prob;
   // the absorption branch writes out[1] and is overwritten below; the kernels'
-  // absorb() is a no-op (acc/kernels/__init__.py:490-492)
+  // absorb() is a no-op (acc/kernels/__init__.py:163-165)
   const double rand = rng.uniform();
   const double xs = distance * rand;
   const double prob = distance * exp(-(mu + sigma) * xs);

@@ -1,6 +1,6 @@
 // mcv_kernel.h -- skeleton of the fused instrument kernels.
 //
-// The reference's generated kernel (acc/run_script.py:320-334) gives each
+// The reference's generated kernel (acc/run_script.py:182-196) gives each
 // thread a contiguous block of neutron indices [tid*npt, min((tid+1)*npt, N))
 // drawn from its own xoroshiro128+ stream, with one global atomic per neutron
 // on a single counter.  Here one lane owns one stream: its state is loaded

@@ -2,7 +2,7 @@
 // sample on the device.
 //
 // The reference renders a shape tree into nested Numba closures that fill a
-// local ts[10] array (acc/geometry/arrow_intersect.py:18-213).  Here the tree is
+// local ts[10] array (acc/geometry/arrow_intersect.py:18-142).  Here the tree is
 // a C++ type (CsgSphere<0>, CsgDifference<0, CsgCylinder<1>, CsgCylinder<2>>,
 // ...) emitted by the fuser, so every root list has a compile-time bound K and
 // stays in registers: the sorted insertion of intersectComposite becomes a
@@ -16,11 +16,11 @@ namespace mcv {
 
 constexpr double CSG_EPS = 1e-10;  // acc/geometry/__init__.py:2
 constexpr int OUTSIDE = 1, ONBORDER = 0, INSIDE = -1;
-constexpr int MAXI = 10;  // acc/geometry/arrow_intersect.py:87
+constexpr int MAXI = 10;  // acc/geometry/arrow_intersect.py:16
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
-// acc/geometry/locate.py:426-463
+// acc/geometry/locate.py:85-122
 __device__ __forceinline__ int locate_sphere(double x, double y, double z, double R) {
   const double d2 = x * x + y * y + z * z;
   if (d2 > (R + CSG_EPS) * (R + CSG_EPS)) return OUTSIDE;
@@ -43,7 +43,7 @@ __device__ __forceinline__ int locate_box(double x, double y, double z, double W
   return ONBORDER;
 }
 
-// acc/geometry/arrow_intersect.py:226-239
+// acc/geometry/arrow_intersect.py:154-168
 __device__ __forceinline__ int intersect_sphere(double x, double y, double z, double vx, double vy, double vz,
                                                 double R, double* ts) {
   const double r0dotv = x * vx + y * vy + z * vz;
@@ -58,7 +58,7 @@ __device__ __forceinline__ int intersect_sphere(double x, double y, double z, do
   return 2;
 }
 
-// acc/geometry/arrow_intersect.py:249-338 (side + top/bottom, combined as the
+// acc/geometry/arrow_intersect.py:178-267 (side + top/bottom, combined as the
 // reference combines them; n == 0 -> no roots)
 __device__ __forceinline__ int intersect_cylinder(double x, double y, double z, double vx, double vy, double vz,
                                                   double R, double H, double* ts) {
@@ -130,7 +130,7 @@ __device__ __forceinline__ int intersect_cylinder(double x, double y, double z, 
   return 2;
 }
 
-// acc/geometry/onbox.py:12-56, evaluated in float64 (the reference uses float32)
+// acc/geometry/onbox.py:8-56, evaluated in float64 (the reference uses float32)
 __device__ __forceinline__ double intersect_rectangle(double rx, double ry, double rz, double vx, double vy, double vz,
                                                       double X, double Y) {
   double t = -rz / vz;
@@ -211,7 +211,7 @@ struct CsgBlock {
 };
 
 // Sorted on-border roots of two children (intersectComposite,
-// acc/geometry/arrow_intersect.py:197-213).  Both child lists are sorted, so
+// acc/geometry/arrow_intersect.py:126-142).  Both child lists are sorted, so
 // the reference's append + sorted-insert yields the sorted multiset of the
 // kept roots; a compare-exchange network over KA+KB slots (dropped roots set
 // to +inf) produces the same list without dynamic indexing.
@@ -370,13 +370,13 @@ __device__ __forceinline__ void calc_time_to_point_of_scattering(const double* t
 // max_scattered, max_ms_loops, max_ms_loops_path1, kernel_data_offset, n0, n1).
 // D = sample data block: CSG nodes, then kernel tables at P[21].
 
-// acc/kernels/__init__.py:483-488 (mu / v * 2200)
+// acc/kernels/__init__.py:156-161 (mu / v * 2200)
 __device__ __forceinline__ double absorption_coeff(const double* P, const Neutron& n) {
   const double v = sqrt(n.vx * n.vx + n.vy * n.vy + n.vz * n.vz);
   return P[0] / v * 2200;
 }
 
-// acc/kernels/isotropic.py:5-21 with p *= sigma (acc/kernels/__init__.py:363-371)
+// acc/kernels/isotropic.py:5-21 with p *= sigma (acc/kernels/__init__.py:36-44)
 template <class T>
 __device__ __forceinline__ void isotropic_scatter(Rng& rng, Neutron& n, double sigma, T& tr) {
   tr.sigma(sigma);

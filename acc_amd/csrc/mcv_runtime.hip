@@ -302,7 +302,7 @@ struct Packed {
 };
 
 // Mirror-normal lengths the guides use at every reflection (guide.py:131-155,
-// guide_tapering.py:240-268: nlen2 = l*l + w*w and sqrt(nlen2)), derived once
+// guide_tapering.py:236-259: nlen2 = l*l + w*w and sqrt(nlen2)), derived once
 // here from the reference-order parameters.  Host IEEE sqrt is correctly
 // rounded, so the values are the reference's.
 void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, ..., l at [4])
@@ -317,8 +317,8 @@ void guide_norms(const double* P, double* out) {  // P = Guide params (ww, hh, .
 // Tapered guide: one 16-double record per segment from the component's
 // 10 x nseg table (w1, w2, h1, h2, ww, hh, whalf, hhalf, lwhalf, lhhalf):
 // [0] w1, [1] hhalf, [2] ww, [3] hh, [4] whalf * l_seg, [5] hhalf * l_seg
-// (the products of cv1 and ch1, guide_tapering.py:202-204), [6] zr = seg *
-// l_seg and [7] zr + l_seg (:167, :205), [8] nlen2 and [9] its sqrt for the
+// (the products of cv1 and ch1, guide_tapering.py:199-201), [6] zr = seg *
+// l_seg and [7] zr + l_seg (:168, :204), [8] nlen2 and [9] its sqrt for the
 // vertical mirrors, [10] and [11] for the horizontal ones, [12] h1, [13] whalf,
 // [14] 1 / [8] and [15] 1 / [10] (correctly rounded).
 constexpr int64_t kTaperRec = 16;
@@ -381,7 +381,7 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
     case MCV_SOURCE_SIMPLE: {
       if ((rc = expect(d, i, 13, 0))) return rc;
       copyp(13);
-      // randvec_target_rect's unit vectors (acc/components/sources/source_simple.py:129-142)
+      // randvec_target_rect's unit vectors (acc/components/sources/source_simple.py:25-34)
       const double target[3] = {0.0, 0.0, d.params[11]}, vertical[3] = {0.0, 1.0, 0.0};
       double p1[3], p2[3];
       cross3(target, vertical, p1);

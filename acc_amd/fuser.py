@@ -2,7 +2,7 @@
 
 The reference text-generates a Python module that inlines every component's
 Numba `propagate` into one @cuda.jit kernel, with applyTransformation between
-components (acc/run_script.py:175-277).  This fuser emits the C++ body of the
+components (acc/run_script.py:37-139, templates :165-289).  This fuser emits the C++ body of the
 same chain (each component's device routine, frame changes between them, and a
 dead-neutron test when MCV_FLAG_GATE_DEAD is set), wraps it in the per-stream
 loop of mcv_kernel.h, and compiles it with hipcc into a code object cached by

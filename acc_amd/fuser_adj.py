@@ -6,7 +6,7 @@ stream), but each component with weight-carrying parameters runs in its
 score-tracking form and each monitor, instead of adding to its histogram, adds
 (G1[bin] p + 2 G2[bin] p^2) times the neutron's score to per-lane sums
 (mcv_adjoint.h).  The reference has no counterpart; the forward chain it
-differentiates is the one of acc/run_script.py:175-277.
+differentiates is the one of acc/run_script.py:37-139 (templates :165-289).
 """
 from .fuser import compile_source
 

@@ -2,10 +2,10 @@
 
 The reference's generated kernel calls the multiple scatterer with a per-thread
 out_neutrons buffer and then runs every downstream component, with its frame
-change, on each output neutron (acc/run_script.py:204-230, template :245-260).
+change, on each output neutron (acc/run_script.py:72-92, template :245-260).
 The same structure is emitted here; the dump records output 0's final state and
 the number of outputs.  As in the reference, one multiple-scattering component
-per chain is supported (run_script.py:211).  When no downstream component
+per chain is supported (run_script.py:73).  When no downstream component
 draws random numbers, the downstream components run on each output as the
 scatterer produces it, and no output list is kept (every draw and every
 output is the reference's; only the order of histogram additions changes).
@@ -14,7 +14,7 @@ Compacted form (the default): in a wavefront most neutrons miss the sample
 (a hollow cylinder 3 cm across in a 10 cm beam), and the scatterer's bounded
 breadth-first loops would run with the few lanes that hit it.  The scatterer's
 first step decides a miss without drawing (its neutron passes as the single
-output, homogeneous_multiple_scatterer.py:214-217 via is_exiting), so phase A
+output, homogeneous_multiple_scatterer.py:218-220 via is_exiting), so phase A
 runs the components before the sample, that test, and for a miss the
 downstream components on the one output; a neutron that will interact is
 queued with its random stream (stream-carrying compaction: the stream pauses

@@ -1,7 +1,7 @@
 """Buffered-mode kernels: a chain (usually one component) over the rows of a
 neutron array, as the reference's `component.process(neutrons)`
 (acc/components/ComponentBase.py:79-107) runs a component's process_kernel
-(acc/components/StochasticComponentBase.py:59-100)."""
+(acc/components/StochasticComponentBase.py:67-100)."""
 from .fuser import _generate, compile_source
 
 

@@ -2,13 +2,13 @@
 
 The reference receives shapes from mcvine's `instrument.nixml` parser and
 renders them into Numba closures with a visitor
-(acc/geometry/arrow_intersect.py:89-195, acc/geometry/locate.py:8-65).  Here a
+(acc/geometry/arrow_intersect.py:18-142, acc/geometry/locate.py:8-65).  Here a
 shape tree is (a) encoded as a node table for the C ABI (MCV_CSG_* in
 include/mcv_acc.h; node 0 is the root) and (b) rendered as the C++ type the
 fuser instantiates (mcv::CsgSphere<i>, mcv::CsgDifference<i, A, B>, ...).
 
 Cylinders have their axis along local z and a height centred on the origin,
-as in the reference (arrow_intersect.py:242-248).
+as in the reference (arrow_intersect.py:171-177).
 """
 import ast
 import operator

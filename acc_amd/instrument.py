@@ -2,9 +2,9 @@
 
 Stands in for what the reference takes from mcvine-core: `mcvine.instrument()`
 and `Instrument.append(component, position, orientation, relativeTo)` (used by
-every instrument script, e.g. tests/acc_sgm_instrument.py:94-119), and
+every instrument script, e.g. tests/acc_sgm_instrument.py:8-33), and
 `neutron_coordinates_transformer.relativePositionOrientation`, which
-acc/run_script.py:279-300 (calcTransformations) calls for each consecutive
+acc/run_script.py:141-162 (calcTransformations) calls for each consecutive
 pair of components.
 
 Orientation convention: Euler angles in degrees, McStas style (mcni follows
@@ -75,7 +75,7 @@ class Geometer:
 
 def relativePositionOrientation(pos1, rot1, pos2, rot2):
     """(offset, rotmat) taking coordinates in frame 1 to frame 2:
-    r2 = rotmat @ (r1 - offset), v2 = rotmat @ v1 (acc/vec3.py:70-75)."""
+    r2 = rotmat @ (r1 - offset), v2 = rotmat @ v1 (acc/vec3.py:71-75)."""
     offset = rot1.T @ (pos2 - pos1)
     rotmat = rot2.T @ rot1
     return offset, rotmat
@@ -105,7 +105,7 @@ def instrument(name="instrument"):
 
 def calcTransformations(instrument):
     """offsets (n-1, 3) and rotmats (n-1, 3, 3) between consecutive components
-    (acc/run_script.py:279-300)."""
+    (acc/run_script.py:141-162)."""
     comps = instrument.components
     g = instrument.geometer
     offsets, rotmats = [], []

@@ -1,7 +1,7 @@
 """Scattering kernels of homogeneous samples (acc/kernels/__init__.py).
 
 The reference bakes a kernel's coefficients into JIT constants
-(ScatterFuncFactory, :335-371); here each kernel packs into the sample's
+(ScatterFuncFactory, :8-145); here each kernel packs into the sample's
 parameter block (include/mcv_acc.h, MCV_SAMPLE_HSS/HMS) so the values stay
 runtime data -- needed to differentiate with respect to them and to scan them
 in batched runs.
@@ -9,7 +9,7 @@ in batched runs.
  * IsotropicKernel: p *= sigma; uniform direction on 4 pi, |v| kept
    (acc/kernels/isotropic.py:5-21).  sigma, mu are the scattering and
    absorption (at 2200 m/s) coefficients in 1/m; mu is scaled by 2200/|v|
-   (acc/kernels/__init__.py:476-488).
+   (acc/kernels/__init__.py:149-161).
  * SQEGridKernel (no reference counterpart): S(Q,E) tabulated on an nQ x nE
    grid of node values at Qmin + i dQ, Emin + j dE (dQ = (Qmax-Qmin)/nQ), with
    the bilinear lookup and index clamping of the reference's only grid kernel

@@ -1,5 +1,5 @@
 """Monte-Carlo normalizer of the I(Q,E) monitor on the device
-(acc/components/monitors/iqe_monitor.py:286-344).
+(acc/components/monitors/iqe_monitor.py:115-174).
 
 The reference shoots N = nQ*nE*1e4 neutrons (E uniform over the monitor's
 energy range, isotropic directions, weight vf/vi) at a copy of the monitor

@@ -53,7 +53,7 @@ def _hollow_cylinder_hss():
 
 def _verdi_taper():
     """Reference tapered-guide test geometry (tests/components/optics/
-    acc_tapered_guide_instrument.py:87-124) with its VERDI segment file."""
+    acc_tapered_guide_instrument.py:17-54) with its VERDI segment file."""
     from .instrument import instrument
     from .components.sources.source_simple import Source_simple
     from .components.optics.guide_tapering import Guide

@@ -130,28 +130,28 @@ def _load_module(script):
 
 
 def loadInstrument(script, **kwds):
-    m = _load_module(script)
-    assert hasattr(m, 'instrument')
-    instrument = m.instrument
-    if not isinstance(instrument, Instrument):
-        assert callable(instrument)  # has to be a method that creates instrument
-        kwds = _getRelevantKwds(instrument, kwds)
-        instrument = instrument(**kwds)
-    return instrument
+    """The instrument an instrument script defines: its `instrument` object,
+    or what calling `instrument` returns, given those of `kwds` that its
+    signature names (the others are reported as unrecognized)."""
+    module = _load_module(script)
+    if not hasattr(module, "instrument"):
+        raise AssertionError("%s does not define `instrument`" % script)
+    target = module.instrument
+    if isinstance(target, Instrument):
+        return target
+    if not callable(target):
+        raise AssertionError("`instrument` in %s is neither an Instrument nor a function" % script)
+    return target(**_take_accepted(target, kwds))
 
 
-def _getRelevantKwds(method, kwds):
-    """return kwd args for the given method, and remove them from the given kwds"""
-    args = inspect.getfullargspec(method).args
-    d = dict()
-    for a in kwds:
-        if a not in args:
-            warnings.warn("Unrecognized kwd: {!r}".format(a))
-    for a in args:
-        if a in kwds:
-            d[a] = kwds[a]
-            del kwds[a]
-    return d
+def _take_accepted(fn, kwds):
+    """Removes from `kwds`, and returns, the keywords fn's signature names;
+    warns about each one it does not name."""
+    accepted = set(inspect.signature(fn).parameters)
+    for name in kwds:
+        if name not in accepted:
+            warnings.warn("Unrecognized kwd: {!r}".format(name))
+    return {name: kwds.pop(name) for name in list(kwds) if name in accepted}
 
 
 def main(argv=None):

@@ -3,7 +3,7 @@
  *
  * The reference (mcvine.acc) has no native boundary: its Python component
  * classes hand `propagate_params` tuples to Numba, which JIT-compiles one fused
- * CUDA kernel per instrument (acc/run_script.py:175-277) and launches it from
+ * CUDA kernel per instrument (acc/run_script.py:37-139, templates :165-289) and launches it from
  * SourceBase.call_process_no_buffer (acc/components/sources/SourceBase.py:36-52).
  * This header is the boundary that replaces that launch: the Python mirror of
  * the component API packs each component into an mcv_comp_desc (same parameter
@@ -41,13 +41,13 @@ extern "C" {
  * MCV_ARM                no parameters (acc/components/optics/arm.py:14-43)
  * MCV_SOURCE_SIMPLE      params[13] = square, width, height, radius, wl_distr,
  *                        Lambda0, dLambda, E0, dE, xw, yh, dist, pmul
- *                        (acc/components/sources/source_simple.py:211-215)
+ *                        (acc/components/sources/source_simple.py:104-108)
  * MCV_GUIDE              params[10] = ww, hh, hw1, hh1, l, R0, Qc, alpha, m, W
  *                        (acc/components/optics/guide.py:45-50)
  * MCV_GUIDE_TAPERING     params[9] = l_seg, R0, Qcx, Qcy, alphax, alphay, mx, my, W
  *                        iparams[1] = nseg; data[10*nseg] = rows w1, w2, h1, h2,
  *                        ww, hh, whalf, hhalf, lwhalf, lhhalf (row-major, 10 x nseg)
- *                        (acc/components/optics/guide_tapering.py:57-75,117-139)
+ *                        (acc/components/optics/guide_tapering.py:73-75,117-139)
  * MCV_SAMPLE_HSS         single scatterer (homogeneous_single_scatterer.py:26-102)
  * MCV_SAMPLE_HMS         multiple scatterer (homogeneous_multiple_scatterer.py:22-342)
  *     iparams[8] = n_shape_nodes, kernel_type, max_scattered_neutrons,
@@ -59,13 +59,13 @@ extern "C" {
  *     data = CSG nodes (MCV_CSG_NODE_SIZE doubles each, root = node 0),
  *            then kernel tables starting at kernel_data_offset.
  * MCV_MON_PSD            params[4] = xmin, xmax, ymin, ymax; iparams[2] = nx, ny
- *                        (acc/components/monitors/psd_monitor.py:66-69)
+ *                        (acc/components/monitors/psd_monitor.py:53-56)
  * MCV_MON_WAVELENGTH     params[6] = xmin, xmax, ymin, ymax, Lmin, Lmax;
- *                        iparams[1] = nchan (wavelength_monitor.py:136-139)
+ *                        iparams[1] = nchan (wavelength_monitor.py:39-42)
  * MCV_MON_IQE            params[9] = Ei, Qmin, Qmax, Emin, Emax, max_angle_in_plane,
  *                        min_angle_in_plane, max_angle_out_of_plane,
  *                        min_angle_out_of_plane; iparams[2] = nQ, nE
- *                        (acc/components/monitors/iqe_monitor.py:214-220)
+ *                        (acc/components/monitors/iqe_monitor.py:43-49)
  * MCV_DET_TUBES          cylindrical array of vertical tubes (no reference
  *                        counterpart): params[8] = radius, height, phi_min,
  *                        phi_max (deg), tof_min, tof_max (s), tube_diameter,
@@ -79,24 +79,24 @@ extern "C" {
 #define MCV_GUIDE_TAPERING 3
 #define MCV_SAMPLE_HSS 4
 #define MCV_SAMPLE_HMS 5
-#define MCV_SLIT 6      /* params[6] = xmin, xmax, ymin, ymax, radius^2, cut (acc/components/optics/slit.py:58-60) */
-#define MCV_BEAMSTOP 7  /* params[6] as MCV_SLIT (acc/components/optics/beamstop.py:58-60) */
+#define MCV_SLIT 6      /* params[6] = xmin, xmax, ymin, ymax, radius^2, cut (acc/components/optics/slit.py:56-58) */
+#define MCV_BEAMSTOP 7  /* params[6] as MCV_SLIT (acc/components/optics/beamstop.py:60-62) */
 #define MCV_MON_PSD 10
 #define MCV_MON_WAVELENGTH 11
 #define MCV_MON_IQE 12
 #define MCV_DET_TUBES 13
 #define MCV_MON_PSD_4PI 14 /* params[5] = phimin, phimax, thetamin, thetamax, radius; iparams = nphi, ntheta
-                              (acc/components/monitors/psd_monitor_4pi.py:47-50) */
+                              (acc/components/monitors/psd_monitor_4pi.py:42-45) */
 #define MCV_MON_POSDIV 15  /* params[5] = xmin, xmax, ymin, ymax, maxdiv; iparams = npos, ndiv
-                              (acc/components/monitors/posdiv_monitor.py:53-56) */
+                              (acc/components/monitors/posdiv_monitor.py:54-57) */
 /* engine-internal sources */
 #define MCV_SRC_IQE_NORMALIZER 20 /* params[3] = Ei, Emin, Emax: the I(Q,E) monitor's
                                      Monte-Carlo normalizer neutrons
-                                     (acc/components/monitors/iqe_monitor.py:317-344) */
+                                     (acc/components/monitors/iqe_monitor.py:146-174) */
 #define MCV_SRC_STORAGE 21        /* iparams[1] = N; data = (N, 10) stored neutrons: neutron i is
                                      row i % N (acc/components/sources/neutronfromstorage.py:70-82) */
 
-/* scattering kernels of samples (acc/kernels/__init__.py:8-22,363-371) */
+/* scattering kernels of samples (acc/kernels/__init__.py:8-22,36-44) */
 #define MCV_KERNEL_ISOTROPIC 1  /* acc/kernels/isotropic.py:5-21 */
 #define MCV_KERNEL_SQE_GRID 2   /* S(Q,E) on a regular grid (no reference) */
 #define MCV_KERNEL_PHONON_DOS 3 /* 3-peak phonon DOS kernel (no reference) */
@@ -150,9 +150,9 @@ const char* mcv_last_error(void);
 int mcv_device_count(void);
 
 /* Build an instrument handle from a component chain.  Replaces the reference's
- * InstrumentWrapper (acc/run_script.py:336-346): offsets[(n-1)*3] and
+ * InstrumentWrapper (acc/run_script.py:198-208): offsets[(n-1)*3] and
  * rotmats[(n-1)*9] are the frame changes between consecutive components
- * (calcTransformations, run_script.py:279-300).  code_object/kernel_name name
+ * (calcTransformations, run_script.py:141-162).  code_object/kernel_name name
  * the fused kernel that the fuser compiled for exactly this chain. */
 mcv_instr* mcv_instr_create(const mcv_comp_desc* comps, int32_t ncomp,
                             const double* offsets, const double* rotmats,
@@ -165,7 +165,7 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params,
                          int32_t n_params);
 
 /* Run ncount neutrons.  Replaces the generated run() and
- * SourceBase.call_process_no_buffer (run_script.py:348-361, SourceBase.py:36-52):
+ * SourceBase.call_process_no_buffer (run_script.py:210-223, SourceBase.py:36-52):
  * the reference partition (ntotalthreads, threads_per_block -> nblocks, npt)
  * and numba's xoroshiro128+ seeding are reproduced, so stream tid runs neutrons
  * [tid*npt, min((tid+1)*npt, ncount)).  Histograms accumulate across calls
@@ -210,7 +210,7 @@ int mcv_instr_set_dump(mcv_instr* h, int64_t first, int64_t count);
 int mcv_instr_read_dump(mcv_instr* h, double* states, int32_t* alive);
 
 /* Buffered mode (acc/components/ComponentBase.py:79-107 `process`, launched as
- * acc/components/StochasticComponentBase.py:21-55 does): `code_object` /
+ * acc/components/StochasticComponentBase.py:21-54 does): `code_object` /
  * `kernel_name` name the chain's buffered-mode kernel (the fuser's process
  * mode for the same chain).  neutrons (n x 10 doubles, x y z vx vy vz s1 s2 t p)
  * are run through the chain in place, stream tid taking rows

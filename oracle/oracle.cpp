@@ -182,7 +182,7 @@ static inline void mXv(const double* m, const double* v, double* o) {
   for (int i = 0; i < 3; ++i) o[i] = m[3 * i + 0] * v[0] + m[3 * i + 1] * v[1] + m[3 * i + 2] * v[2];
 }
 
-// acc/neutron.py:18-23 abs2rel; acc/vec3.py:70-75 _abs2rel: r' = R (r - o); v' = R v
+// acc/neutron.py:18-22 abs2rel; acc/vec3.py:71-75 _abs2rel: r' = R (r - o); v' = R v
 static inline void apply_transformation(double* n, const double* rot, const double* off) {
   double rt[3], vt[3], tmp[3];
   copy3(n, rt);
@@ -194,7 +194,7 @@ static inline void apply_transformation(double* n, const double* rot, const doub
   mXv(rot, vt, n + 3);
 }
 
-// acc/neutron.py:67-74
+// acc/neutron.py:44-51 prop_dt_inplace
 static inline void prop_dt_inplace(double* n, double dt) {
   double x = n[0], y = n[1], z = n[2], vx = n[3], vy = n[4], vz = n[5], t = n[8];
   n[0] = x + vx * dt;
@@ -203,7 +203,7 @@ static inline void prop_dt_inplace(double* n, double dt) {
   n[8] = t + dt;
 }
 
-// acc/neutron.py:83-89
+// acc/neutron.py:60-66 prop_z0
 static inline void prop_z0(const double* n, double& x, double& y, double& t) {
   double dt = -n[2] / n[5];
   x = n[0] + n[3] * dt;
@@ -211,7 +211,7 @@ static inline void prop_z0(const double* n, double& x, double& y, double& t) {
   t = n[8] + dt;
 }
 
-static inline bool is_moving(const double* n) {  // acc/neutron.py:60-65
+static inline bool is_moving(const double* n) {  // acc/neutron.py:38-42
   double v = std::sqrt(n[3] * n[3] + n[4] * n[4] + n[5] * n[5]);
   return v > 1.e-6;
 }
@@ -281,7 +281,7 @@ static void source_simple(Rng& rng, double* n, const double* P) {
   n[9] = pmul * solidangle;
 }
 
-// I(Q,E)-monitor normalizer neutrons (acc/components/monitors/iqe_monitor.py:317-344,
+// I(Q,E)-monitor normalizer neutrons (acc/components/monitors/iqe_monitor.py:146-174,
 // makeNeutrons), one neutron per call: E, cos(theta), phi drawn in that order.
 static void iqe_normalizer_source(Rng& rng, double* n, const double* P) {
   double Ei = P[0], Emin = P[1], Emax = P[2];
@@ -515,14 +515,14 @@ static void guide_tapering(double* n, const double* P, int64_t nseg, const doubl
 // acc/geometry/__init__.py:2, location.py:1-3
 static const double EPS = 1e-10;
 enum { OUTSIDE = 1, ONBORDER = 0, INSIDE = -1 };
-static const int MAXI = 10;  // acc/geometry/arrow_intersect.py:87 max_intersections
+static const int MAXI = 10;  // acc/geometry/arrow_intersect.py:16 max_intersections
 
 struct Shape {
   const double* nodes;  // MCV_CSG_NODE_SIZE doubles per node
   int n;
 };
 
-// acc/geometry/locate.py:426-463
+// acc/geometry/locate.py:85-122 (sphere, cylinder, block)
 static int locate_sphere(double x, double y, double z, double R) {
   double dist2 = x * x + y * y + z * z;
   if (dist2 > (R + EPS) * (R + EPS)) return OUTSIDE;
@@ -545,7 +545,7 @@ static int locate_box(double x, double y, double z, double W, double H, double D
   return ONBORDER;
 }
 
-// acc/geometry/locate.py:408-491 (union, difference, intersection)
+// acc/geometry/locate.py:68-82,125-151 (union, difference, intersection)
 static int locate(const Shape& s, int idx, double x, double y, double z) {
   const double* nd = s.nodes + MCV_CSG_NODE_SIZE * idx;
   int type = (int)nd[0];
@@ -586,7 +586,7 @@ static int locate(const Shape& s, int idx, double x, double y, double z) {
   return OUTSIDE;
 }
 
-// acc/geometry/_utils.py:9-37
+// acc/geometry/_utils.py:4-32
 static int insert_into_sorted_list(double d, double* l, int N, int len) {
   if (N >= len) return N;
   if (N == 0) {
@@ -618,7 +618,7 @@ static int insert_into_sorted_list(double d, double* l, int N, int len) {
   return N + 1;
 }
 
-// acc/geometry/arrow_intersect.py:226-239
+// acc/geometry/arrow_intersect.py:154-168
 static void intersect_sphere(double x, double y, double z, double vx, double vy, double vz, double R, double& t1,
                              double& t2) {
   double r0dotv = x * vx + y * vy + z * vz;
@@ -634,7 +634,7 @@ static void intersect_sphere(double x, double y, double z, double vx, double vy,
   t2 = -(r0dotv - sq) / v2;
 }
 
-// acc/geometry/arrow_intersect.py:249-275
+// acc/geometry/arrow_intersect.py:178-204
 static int cyl_side(double x, double y, double z, double vx, double vy, double vz, double R, double H, double& ta,
                     double& tb) {
   double a = vx * vx + vy * vy, b = x * vx + y * vy, c = x * x + y * y - R * R;
@@ -667,7 +667,7 @@ static int cyl_side(double x, double y, double z, double vx, double vy, double v
   return 0;
 }
 
-// acc/geometry/arrow_intersect.py:284-310
+// acc/geometry/arrow_intersect.py:213-239
 static int cyl_topbottom(double x, double y, double z, double vx, double vy, double vz, double R, double H,
                          double& ta, double& tb) {
   ta = tb = NAN;
@@ -704,7 +704,7 @@ static int cyl_topbottom(double x, double y, double z, double vx, double vy, dou
 static inline double pymin(double a, double b) { return b < a ? b : a; }  // Python min(a, b)
 static inline double pymax(double a, double b) { return b > a ? b : a; }  // Python max(a, b)
 
-// acc/geometry/arrow_intersect.py:312-338 (n==0 falls through to the n==4 branch: all NaN)
+// acc/geometry/arrow_intersect.py:241-267 (n==0 falls through to the n==4 branch: all NaN)
 static void intersect_cylinder(double x, double y, double z, double vx, double vy, double vz, double R, double H,
                                double& o1, double& o2) {
   double t1, t2, t3, t4;
@@ -729,7 +729,7 @@ static void intersect_cylinder(double x, double y, double z, double vx, double v
   }
 }
 
-// acc/geometry/onbox.py:12-56 (the reference computes this in float32; restated in float64)
+// acc/geometry/onbox.py:8-56 (the reference computes this in float32; restated in float64)
 static double intersect_rectangle(double rx, double ry, double rz, double vx, double vy, double vz, double X, double Y) {
   double t = -rz / vz;
   double r1x = rx + vx * t, r1y = ry + vy * t;
@@ -766,7 +766,7 @@ static void intersect_box(double x, double y, double z, double vx, double vy, do
   }
 }
 
-// ArrowIntersectFuncFactory (acc/geometry/arrow_intersect.py:96-213)
+// ArrowIntersectFuncFactory (acc/geometry/arrow_intersect.py:18-123)
 static int intersect(const Shape& s, int idx, double x, double y, double z, double vx, double vy, double vz,
                      double* ts) {
   const double* nd = s.nodes + MCV_CSG_NODE_SIZE * idx;
@@ -792,7 +792,7 @@ static int intersect(const Shape& s, int idx, double x, double y, double z, doub
       ts[1] = t2;
       return 2;
     default: {
-      // intersectComposite (acc/geometry/arrow_intersect.py:197-213)
+      // intersectComposite (acc/geometry/arrow_intersect.py:126-142)
       double ts1[MAXI], ts2[MAXI];
       int N1 = intersect(s, (int)nd[4], x, y, z, vx, vy, vz, ts1);
       int N2 = intersect(s, (int)nd[5], x, y, z, vx, vy, vz, ts2);
@@ -1001,14 +1001,14 @@ struct Sample {
   int max_scattered, max_ms_loops, max_ms_loops_path1;
 };
 
-// acc/kernels/__init__.py:476-488: sigma constant; mu/v*2200
+// acc/kernels/__init__.py:149-161: sigma constant; mu/v*2200
 static inline double calc_scattering_coeff(const Sample& S, const double* n) { (void)n; return S.sigma; }
 static inline double calc_absorption_coeff(const Sample& S, const double* n) {
   double v = length(n + 3);
   return S.mu / v * 2200;
 }
 
-// acc/kernels/isotropic.py:5-21 (+ acc/kernels/__init__.py:363-371: p *= sigma)
+// acc/kernels/isotropic.py:5-21 (+ acc/kernels/__init__.py:36-44: p *= sigma)
 static void isotropic_scatter(Rng& rng, double* n, double sigma) {
   n[9] *= sigma;
   double cos_t = (double)uniform_f32(rng) * 2 - 1;
@@ -1182,7 +1182,7 @@ static int interactM1(Rng& rng, const Sample& S, double* out, double* n) {
   propagate_to_next_exiting_surface(S.shape, transmitted);
   transmitted[9] *= transmission_prob;
   // absorption branch: temp = out[1] is overwritten by the scattered neutron
-  // below and the kernels' absorb() is a no-op (acc/kernels/__init__.py:490-492)
+  // below and the kernels' absorb() is a no-op (acc/kernels/__init__.py:163-165)
   float rand = uniform_f32(rng);
   double xs = distance * (double)rand;
   double prob = distance * TR_EXP(-(mu + sigma) * xs);
@@ -1297,7 +1297,7 @@ static int scatterM(Rng& rng, const Sample& S, double* out, int out_len, double*
 }
 
 // ---------------------------------------------------------------- slit, beamstop
-// acc/components/optics/slit.py:72-105 and beamstop.py:70-103 (absorb = p -> -1)
+// acc/components/optics/slit.py:71-108 and beamstop.py:75-112 (absorb = p -> -1)
 static void aperture(double* n, const double* P, bool beamstop) {
   double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], radius_squared = P[4], cut = P[5];
   double x = n[0], y = n[1], z = n[2], vx = n[3], vy = n[4], vz = n[5];
@@ -1361,7 +1361,7 @@ static inline void hist_event(double* out, int64_t nb, int64_t b, double p) {
 }
 
 // ---------------------------------------------------------------- monitors (R14-R17)
-// acc/components/monitors/psd_monitor.py:79-96
+// acc/components/monitors/psd_monitor.py:66-83
 static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny, double* out) {
   double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3];
   double t0 = n[8], x, y, t;
@@ -1378,7 +1378,7 @@ static void psd_monitor(const double* n, const double* P, int64_t nx, int64_t ny
   hist_event(out, nb, b, p);
 }
 
-// acc/components/monitors/posdiv_monitor.py:70-95 (NaN rejected and indices
+// acc/components/monitors/posdiv_monitor.py:67-89 (NaN rejected and indices
 // clamped, as for the PSD monitor)
 static void posdiv_monitor(const double* n, const double* P, int64_t npos, int64_t ndiv, double* out) {
   double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], maxdiv = P[4];
@@ -1398,7 +1398,7 @@ static void posdiv_monitor(const double* n, const double* P, int64_t npos, int64
   hist_event(out, nb, b, p);
 }
 
-// acc/components/monitors/psd_monitor_4pi.py:59-81 with
+// acc/components/monitors/psd_monitor_4pi.py:56-79 with
 // acc/geometry/arrow_intersect.py:155-168 (moves the neutron onto the sphere;
 // a line that misses the sphere is not counted here)
 static void psd4pi_monitor(double* n, const double* P, int64_t nphi, int64_t ntheta, double* out) {
@@ -1426,7 +1426,7 @@ static void psd4pi_monitor(double* n, const double* P, int64_t nphi, int64_t nth
   hist_event(out, nb, b, p);
 }
 
-// acc/components/monitors/wavelength_monitor.py:149-171
+// acc/components/monitors/wavelength_monitor.py:52-74
 static void wavelength_monitor(const double* n, const double* P, int64_t nchan, double* out) {
   double xmin = P[0], xmax = P[1], ymin = P[2], ymax = P[3], Lmin = P[4], Lmax = P[5];
   double t0 = n[8], x, y, t;
@@ -1443,7 +1443,7 @@ static void wavelength_monitor(const double* n, const double* P, int64_t nchan, 
   hist_event(out, nchan, iL, p);
 }
 
-// acc/components/monitors/iqe_monitor.py:251-284
+// acc/components/monitors/iqe_monitor.py:80-113
 static void iqe_monitor(const double* n, const double* P, int64_t nQ, int64_t nE, double* out) {
   double Ei = P[0], Qmin = P[1], Qmax = P[2], Emin = P[3], Emax = P[4], max_in = P[5], min_in = P[6],
          max_out = P[7], min_out = P[8];
@@ -1572,7 +1572,7 @@ static void run_component(const Chain& C, int i, Rng& rng, double* n, double* hi
 static inline bool dead(const Chain& C, const double* n) { return C.gate && !(n[9] > 0); }
 
 // Components [from, end) on one neutron (the tail of the fused kernel body,
-// acc/run_script.py:197-231).  Returns true if it reached the end alive.
+// generated by acc/run_script.py:56-93).  Returns true if it reached the end alive.
 static bool run_tail(const Chain& C, int from, Rng& rng, double* n, double* hist, int64_t nidx) {
   int nc = (int)C.comps.size();
   for (int i = from; i < nc; ++i) {
@@ -1585,7 +1585,7 @@ static bool run_tail(const Chain& C, int from, Rng& rng, double* n, double* hist
 
 // One neutron through the whole chain; final state left in n.  For a chain
 // with a multiple-scattering sample, the sample's outputs each run the rest of
-// the chain (run_script.py:225-230); n then holds output 0's final state and
+// the chain (the loop acc/run_script.py:87-92 generates); n then holds output 0's final state and
 // the return value is the number of outputs.
 static int run_neutron(const Chain& C, Rng& rng, double* n, double* hist, std::vector<double>& msbuf,
                        bool& alive, int64_t nidx) {

@@ -80,9 +80,9 @@ def instrument():
 
 def test_saved_histograms_are_the_scaled_sums(tmp_path, monkeypatch):
     """What run() saves is the reference's finalization of the raw sums
-    (acc/run_script.py:339-343 saveMonitorOutputs with scale_factor 1/ncount;
-    psd_monitor.py:53-59 getHistogram: I = sum p * s, E2 = sum p^2 * s^2;
-    iqe_monitor.py:57-60 getHistogram: that histogram divided by the
+    (acc/run_script.py:291-295 saveMonitorOutputs with scale_factor 1/ncount;
+    psd_monitor.py:58-64 getHistogram: I = sum p * s, E2 = sum p^2 * s^2;
+    iqe_monitor.py:67-70 getHistogram: that histogram divided by the
     Monte-Carlo normalizer, errors propagated as for a quotient), checked
     against the CPU oracle's raw sums for the same neutrons; a run split into
     launches by buffer_size saves the same counts."""

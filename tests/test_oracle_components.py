@@ -253,7 +253,7 @@ def test_posdiv_monitor_closed_form():
 def test_psd4pi_monitor_closed_form():
     """A neutron leaving the origin in direction (phi, theta) reaches the sphere
     at radius R after R / |v|; bins follow the reference's formulas
-    (psd_monitor_4pi.py:70-75, theta rounding half up)."""
+    (psd_monitor_4pi.py:67-74, theta rounding half up)."""
     from acc_amd.components.monitors.psd_monitor_4pi import PSD_monitor_4Pi
     R = 1.5
     m = PSD_monitor_4Pi('m4', radius=R, nphi=36, ntheta=18)

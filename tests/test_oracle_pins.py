@@ -7,11 +7,11 @@ Each expectation comes from geometry or kinematics computed here from the
 neutron's own state (Q and E recomputed from the velocities, hit points from
 the ray equation), so that a swapped axis, a shifted node index, a swapped
 parameter or a wrong normal fails: none of them retypes the oracle's formula.
-Patterns: acc/kernels/SANS2D_ongrid.py:34-56 (the grid lookup),
+Patterns: acc/kernels/SANS2D_ongrid.py:42-55 (the grid lookup),
 tests/kernels/test_E_Q.py:25-34 and tests/kernels/test_constant_qe.py:55-65
 ((Q, E) recovered from the final velocity), acc/components/monitors/
 wavelength_monitor.py:52-74, iqe_monitor.py:146-174,
-acc/components/optics/guide_tapering.py:193-284."""
+acc/components/optics/guide_tapering.py:189-284."""
 import math
 
 import numpy as np
