@@ -97,6 +97,15 @@ class DifferentiableRun:
         self.last_forward_stats = self.h.run(self.ncount, self.seed, self.ntotalthreads, self.tpb, self.flags)
         return np.concatenate([self.h.read_monitor(i)[1] for i in self.monitors])
 
+    def counts(self):
+        """Concatenated neutron-count histograms of the selected monitors after
+        the last forward(): the weight-carrying parameters do not move any
+        neutron, so equal counts at two parameter values certify that no
+        neutron changed its fate (a kill decision at the reflectivity cutoff,
+        which with dead neutrons dropped also changes the random numbers of the
+        stream's later neutrons) between them."""
+        return np.concatenate([self.h.read_monitor(i)[0] for i in self.monitors])
+
     def vjp(self, g, theta=None, dense=False):
         """g^T dH/dtheta for an upstream gradient g of forward()'s output
         (and, with dense=True, the gradient with respect to the S(Q,E) grid)."""

@@ -5,9 +5,13 @@ torch.autograd wrapper, and a fit of perturbed parameters back to a target.
 
 The adjoint has no reference counterpart; what pins it is that for fixed
 random streams the histograms are smooth functions of the weight-carrying
-parameters, so finite differences of the forward simulation (GPU kernel and
-CPU oracle, each a restatement of the reference's forward algorithm) give
-the same derivatives."""
+parameters wherever no neutron's fate changes, so finite differences of the
+forward simulation (GPU kernel and CPU oracle, each a restatement of the
+reference's forward algorithm) give the same derivatives.  At 1e7 neutrons a
+few neutrons sit within a 1e-6 step of the supermirror cutoff, where a
+reflectivity drops to zero and the neutron dies (and, with dead neutrons
+dropped, its stream's later neutrons draw other numbers): the difference
+steps are certified by equal monitor counts and cut when a fate flips."""
 import copy
 
 import numpy as np
@@ -27,18 +31,43 @@ def _scale(f0, theta):
     return np.maximum(np.abs(f0) / np.maximum(np.abs(theta), 1e-300), 1e-300)
 
 
-def _fd(fun, theta, j):
-    h = H_REL * abs(theta[j])
+def _fd(fun, theta, j, h_rel=H_REL):
+    h = h_rel * abs(theta[j])
     tp, tm = theta.copy(), theta.copy()
     tp[j] += h
     tm[j] -= h
     return (fun(tp) - fun(tm)) / (tp[j] - tm[j])
 
 
+def _fd_certified(run, g, theta, j):
+    """Central difference of g^T H in theta[j] whose two runs have identical
+    monitor counts.  The histograms are smooth in the weight-carrying
+    parameters except where a neutron's fate flips: a reflectivity that crosses
+    the supermirror cutoff ((Q - m Qc) / W = 10, R -> 0) kills the neutron,
+    which then draws no more random numbers, so with dead neutrons dropped the
+    rest of its stream runs on other numbers.  Such a flip inside +-h changes
+    the counts; the step is then cut tenfold.  Returns (difference, h_rel)."""
+    h_rel = H_REL
+    for _ in range(4):
+        h = h_rel * abs(theta[j])
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += h
+        tm[j] -= h
+        fp = float(g @ run.forward(tp))
+        cp = run.counts()
+        fm = float(g @ run.forward(tm))
+        cm = run.counts()
+        if np.array_equal(cp, cm):
+            return (fp - fm) / (tp[j] - tm[j]), h_rel
+        h_rel /= 10
+    raise AssertionError("parameter %d: neutron fates differ at every step down to %g" % (j, h_rel * 10))
+
+
 @pytest.fixture(scope="module")
 def c5run():
     print("\n[adjoint] building c5", flush=True)
-    run = DifferentiableRun(configs.c5(), configs.C5_PARAMS, ncount=400000, seed=7)
+    # the configured 1e7 neutrons
+    run = DifferentiableRun(configs.c5(), configs.C5_PARAMS, ncount=int(1e7), seed=7)
     yield run
 
 
@@ -47,14 +76,15 @@ def test_adjoint_matches_gpu_crn_fd(c5run):
     theta = run.theta0()
     g = np.random.default_rng(1).uniform(0.5, 1.5, sum(run.sizes))
     adj = run.vjp(g, theta)
-    f = lambda t: float(g @ run.forward(t))
-    f0 = f(theta)
+    f0 = float(g @ run.forward(theta))
     assert f0 > 0
-    fd = np.array([_fd(f, theta, j) for j in range(len(theta))])
+    fd, steps = zip(*[_fd_certified(run, g, theta, j) for j in range(len(theta))])
+    fd = np.array(fd)
     run.set_theta(theta)
     rel = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-9 * _scale(f0, theta))
     for j, p in enumerate(configs.C5_PARAMS):
-        print("[adjoint] %-40s adj % .9e fd % .9e rel %.2e" % (p, adj[j], fd[j], rel[j]), flush=True)
+        print("[adjoint] %-40s adj % .9e fd % .9e (h %.0e) rel %.2e" % (p, adj[j], fd[j], steps[j], rel[j]),
+              flush=True)
     assert np.all(rel <= TOL), rel
 
 
@@ -138,12 +168,14 @@ def test_dense_grid_gradient():
 
 
 def test_autograd_and_fit():
-    """torch gradients equal the adjoint's; a fit from perturbed peak
-    parameters returns to the target's values (common random numbers)."""
+    """torch gradients equal the adjoint's; a fit of all 14 parameters (nine
+    peak parameters, the two cross-sections, the shared guide coating m, W,
+    alpha) from +-10% perturbed values returns to the target's values (common
+    random numbers)."""
     import torch
     from acc_amd.autograd import histograms
-    params = configs.C5_PARAMS[:9]
-    N = 200000
+    params = configs.C5_PARAMS
+    N = 1000000
     target_run = DifferentiableRun(configs.c5(), params, ncount=N, seed=11)
     truth = target_run.theta0()
     target = torch.from_numpy(target_run.forward(truth))
@@ -165,9 +197,13 @@ def test_autograd_and_fit():
     gH = 2 * (H - target.numpy()) / scale
     direct = run.vjp(gH, (t0 * (1 + x)).detach().numpy()) * truth
     np.testing.assert_allclose(x.grad.numpy(), direct, rtol=1e-10, atol=1e-300)
-    first = float(loss)
-    opt = torch.optim.LBFGS([x], lr=1, max_iter=60, tolerance_grad=1e-14, tolerance_change=1e-16,
-                            history_size=20, line_search_fn="strong_wolfe")
+    first = float(loss.detach())
+    # the coating, the cross-sections and the peak amplitudes are nearly
+    # degenerate in the histograms (sigma_s and A_k enter as products; sigma_a
+    # differs from sigma_s only by the 1/v of absorption), so the fit needs a
+    # long curvature history
+    opt = torch.optim.LBFGS([x], lr=1, max_iter=3000, max_eval=6000, tolerance_grad=0, tolerance_change=0,
+                            history_size=100, line_search_fn="strong_wolfe")
 
     def closure():
         opt.zero_grad()
@@ -176,7 +212,9 @@ def test_autograd_and_fit():
         return lo
 
     opt.step(closure)
-    final = float(loss_fn())
+    final = float(loss_fn().detach())
     print("[adjoint] fit loss %.3e -> %.3e, max |x| %.2e" % (first, final, float(x.abs().max())), flush=True)
-    assert final < 1e-6 * first
-    assert float(x.abs().max()) < 1e-3
+    for name, v in zip(params, x.detach().numpy()):
+        print("[adjoint]   %-45s relative offset %.2e" % (name, v), flush=True)
+    assert final < 1e-20 * first
+    assert float(x.abs().max()) < 1e-8

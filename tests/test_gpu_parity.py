@@ -64,6 +64,8 @@ CASES = [
     ("csg-intersection", True, 200000, 4096),
     ("csg-difference", True, 200000, 4096),
     ("c3-small", True, 100000, 4096),
+    # the configured 400 x 128 x 500 tube array (a 614 MB histogram, not replicated)
+    ("c3", True, 100000, 8192),
     ("c3-small", False, 50000, 2048),
     ("hms-ref", True, 30000, 2048),
     ("hms-sample", True, 30000, 2048),
