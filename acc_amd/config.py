@@ -18,6 +18,11 @@ kernel_block = 256
 kernel_waves_per_eu = None
 # random streams per lane in compacted kernels whose later stages draw
 kernel_streams_per_lane = 4
+# workgroup size of plain (uncompacted) chains, whose monitors add through the
+# LDS bin cache and whose grid holds the resident workgroups (c1, 1e7
+# neutrons, 2048 cache slots: 0.96 ms at 256, 0.90 at 512, 1.07 at 1024;
+# 1.93 ms without the cache)
+plain_kernel_block = 512
 # launch shape of chains with a multiple-scattering sample: their scatterer
 # needs the whole register file (occupancy 1 unconstrained); a target of 2
 # waves per SIMD trades spills for latency hiding (c3, 1e8 neutrons: 78 ms at

@@ -81,6 +81,60 @@ __device__ __forceinline__ void dump_neutron(const LaunchArgs& A, int64_t idx, c
 // to the histogram.
 extern __shared__ double mcv_lds_hist[];
 
+// Global destination of monitor comp's (3, nbins) block for this workgroup:
+// its replica when replicated, else the histogram itself.
+__device__ __forceinline__ double* hist_block(const LaunchArgs& A, int comp, int64_t nbins) {
+  if (A.roff[comp] >= 0) {
+    const int64_t r = (int64_t)(blockIdx.x & (unsigned)A.rmask[comp]);
+    return A.R + A.roff[comp] + r * 3 * nbins;
+  }
+  return A.H + A.hoff[comp];
+}
+
+// ------------------------------------------------------------------ LDS bin cache
+// A direct beam puts thousands of events on each of a few thousand bins, and
+// every event of a large monitor would otherwise be three device-scope atomics
+// performed past the XCD's L2.  A workgroup keeps the bins its events hit in
+// an open-addressed LDS table: key = comp << 26 | bin (bins < 2^26), two
+// probes, a slot claimed with one LDS compare-and-swap; an event whose two
+// slots hold other bins goes to the global block as before.  The first claim
+// of a component also records its global block and bin count in the table's
+// header, from which the flush finds every claimed bin's destination.
+// Layout (doubles from cache_off): header [2 * MCV_MAX_COMP], keys
+// [cache_slots / 2] (int32), then N, sum p and sum p^2 [cache_slots each].
+constexpr int kCacheHeader = 2 * MCV_MAX_COMP;
+constexpr int32_t kCacheEmpty = -1;
+
+__device__ __forceinline__ bool cache_add(const LaunchArgs& A, int comp, int64_t nbins, int64_t b, double p) {
+  double* base = mcv_lds_hist + A.cache_off;
+  int32_t* keys = reinterpret_cast<int32_t*>(base + kCacheHeader);
+  double* val = base + kCacheHeader + (A.cache_slots >> 1);
+  const int32_t key = (int32_t)(((uint32_t)comp << 26) | (uint32_t)b);
+  const uint32_t mask = (uint32_t)A.cache_slots - 1u;
+  uint32_t s = ((uint32_t)key * 2654435761u) >> 7;
+#pragma unroll
+  for (int probe = 0; probe < 2; ++probe, s += 1u) {
+    const int j = (int)(s & mask);
+    int32_t k = keys[j];
+    if (k == kCacheEmpty) {
+      k = atomicCAS(keys + j, kCacheEmpty, key);
+      if (k == kCacheEmpty) {
+        k = key;
+        // header of comp: destination block and bin count (the same values from every claimer)
+        reinterpret_cast<double**>(base)[comp] = hist_block(A, comp, nbins);
+        reinterpret_cast<int64_t*>(base)[MCV_MAX_COMP + comp] = nbins;
+      }
+    }
+    if (k == key) {
+      atomicAdd(val + j, 1.0);
+      atomicAdd(val + A.cache_slots + j, p);
+      atomicAdd(val + 2 * A.cache_slots + j, p * p);
+      return true;
+    }
+  }
+  return false;
+}
+
 __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_t nbins, int64_t b, double p) {
   const int lo = A.loff[comp];
   if (lo >= 0) {
@@ -88,24 +142,48 @@ __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_
     atomicAdd(L + b, 1.0);
     atomicAdd(L + nbins + b, p);
     atomicAdd(L + 2 * nbins + b, p * p);
-  } else if (A.roff[comp] >= 0) {
-    const int64_t r = (int64_t)(blockIdx.x & (unsigned)A.rmask[comp]);
-    hist_add3(A.R + A.roff[comp] + r * 3 * nbins, nbins, b, p);
+  } else if (((A.cmask >> comp) & 1u) && cache_add(A, comp, nbins, b, p)) {
+    return;
   } else {
-    hist_add3(A.H + A.hoff[comp], nbins, b, p);
+    hist_add3(hist_block(A, comp, nbins), nbins, b, p);
   }
 }
 
-// All threads of the workgroup call these (barriers when A.lds_nseg > 0).
+// All threads of the workgroup call these (barriers when LDS histograms or the cache are in use).
 __device__ __forceinline__ void lds_hist_zero(const LaunchArgs& A) {
-  if (A.lds_nseg == 0) return;
+  if (A.lds_nseg == 0 && A.cache_slots == 0) return;
   for (int j = threadIdx.x; j < A.lds_total; j += blockDim.x) mcv_lds_hist[j] = 0;
+  if (A.cache_slots > 0) {
+    double* base = mcv_lds_hist + A.cache_off;
+    int32_t* keys = reinterpret_cast<int32_t*>(base + kCacheHeader);
+    double* val = base + kCacheHeader + (A.cache_slots >> 1);
+    for (int j = threadIdx.x; j < A.cache_slots; j += blockDim.x) keys[j] = kCacheEmpty;
+    for (int j = threadIdx.x; j < 3 * A.cache_slots; j += blockDim.x) val[j] = 0;
+  }
   __syncthreads();
 }
 
+__device__ __forceinline__ void cache_flush(const LaunchArgs& A) {
+  double* base = mcv_lds_hist + A.cache_off;
+  const int32_t* keys = reinterpret_cast<const int32_t*>(base + kCacheHeader);
+  const double* val = base + kCacheHeader + (A.cache_slots >> 1);
+  for (int j = threadIdx.x; j < A.cache_slots; j += blockDim.x) {
+    const int32_t key = keys[j];
+    if (key == kCacheEmpty) continue;
+    const int comp = (int)((uint32_t)key >> 26);
+    const int64_t b = (int64_t)((uint32_t)key & 0x3ffffffu);
+    double* G = reinterpret_cast<double* const*>(base)[comp];
+    const int64_t nb = reinterpret_cast<const int64_t*>(base)[MCV_MAX_COMP + comp];
+    unsafeAtomicAdd(G + b, val[j]);
+    unsafeAtomicAdd(G + nb + b, val[A.cache_slots + j]);
+    unsafeAtomicAdd(G + 2 * nb + b, val[2 * A.cache_slots + j]);
+  }
+}
+
 __device__ __forceinline__ void lds_hist_flush(const LaunchArgs& A) {
-  if (A.lds_nseg == 0) return;
+  if (A.lds_nseg == 0 && A.cache_slots == 0) return;
   __syncthreads();
+  if (A.cache_slots > 0) cache_flush(A);
   // constant indices only: the batch kernel keeps its LaunchArgs copy out of
   // scratch memory only if no field is indexed at run time
 #pragma unroll
@@ -164,6 +242,33 @@ __device__ __forceinline__ void run_lane(const LaunchArgs& A, int64_t tid) {
     dump_neutron(A, i, n, alive);
   }
   count_processed(A.counter, end - idx);
+  lds_hist_flush(A);
+}
+
+// The plain kernel: workgroup g runs the blocks of MCV_BLOCK streams g, g +
+// gridDim.x, ... of the launch's range, so that with a grid of the resident
+// workgroups their LDS histograms and bin caches serve many streams before
+// one flush (with one block per workgroup it is run_lane).
+template <class Chain>
+__device__ __forceinline__ void run_lanes(const LaunchArgs& A) {
+  lds_hist_zero(A);
+  const int64_t last = A.tid_end < A.nthreads ? A.tid_end : A.nthreads;
+  int64_t mine = 0;
+  Neutron n;
+  n.s1 = 0;
+  n.s2 = 0;
+  for (int64_t first = A.tid_base + (int64_t)blockIdx.x * MCV_BLOCK; first < last;
+       first += (int64_t)gridDim.x * MCV_BLOCK) {
+    Rng rng;
+    int64_t idx, end;
+    stream_range(A, first + threadIdx.x, rng, idx, end);
+    for (int64_t i = idx; i < end; ++i) {
+      const int alive = (int)Chain::body(rng, n, A, i);
+      dump_neutron(A, i, n, alive);
+    }
+    mine += end - idx;
+  }
+  count_processed(A.counter, mine);
   lds_hist_flush(A);
 }
 
@@ -565,9 +670,9 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 }  // namespace mcv
 
 #define MCV_DEFINE_KERNELS_PLAIN(NAME, CHAIN)                                                  \
-  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {     \
-    const int64_t tid = A.tid_base + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;          \
-    mcv::run_lane<CHAIN>(A, tid);                                                             \
+  extern "C" __device__ int mcv_lds_cache = 1;                                                \
+  extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
+    mcv::run_lanes<CHAIN>(A);                                                                 \
   }                                                                                           \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME##_batch(const mcv::BatchArgs B) { \
     mcv::LaunchArgs A;                                                                        \

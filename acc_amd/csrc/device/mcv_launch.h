@@ -44,6 +44,13 @@ struct LaunchArgs {
   double* R;
   int64_t roff[MCV_MAX_COMP];
   int32_t rmask[MCV_MAX_COMP];
+  // LDS bin cache: monitors whose bit is set in cmask add through an
+  // open-addressed table of cache_slots (a power of two) bins in dynamic LDS,
+  // starting cache_off doubles in (after the privatised histograms); claimed
+  // bins are flushed to their histogram (or replica) when the workgroup ends.
+  // cache_slots = 0: no cache.
+  int32_t cache_slots, cache_off;
+  uint32_t cmask;
   // buffered mode: neutron i of the run is row i of io (N, 10), updated in
   // place; a multiple-scattering sample writes its io_m outputs per neutron to
   // io_out rows [i*io_m, (i+1)*io_m)

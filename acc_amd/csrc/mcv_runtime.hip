@@ -522,6 +522,33 @@ int hist_replicas() {
 }
 constexpr int64_t kReplicaBytes = int64_t(256) << 20;
 
+// Slots of the LDS bin cache of plain kernels (MCV_LDS_CACHE_SLOTS from the
+// environment, rounded down to a power of two, at least 64; 0 turns it off),
+// default 2048 (57 KB of LDS per workgroup).  Monitors of at most 2^20 bins
+// that are not privatised whole use it.
+int lds_cache_slots() {
+  const char* e = std::getenv("MCV_LDS_CACHE_SLOTS");
+  long long v = (e && *e) ? std::atoll(e) : 2048;
+  if (v < 64) return 0;
+  int s = 64;
+  while ((long long)s * 2 <= std::min<long long>(v, 8192)) s *= 2;
+  return s;
+}
+constexpr int64_t kCacheMaxBins = int64_t(1) << 20;
+constexpr int kCacheHeaderDoubles = 2 * MCV_MAX_COMP;
+
+// Dynamic LDS of a launch, in doubles: privatised histograms, then the bin cache.
+int64_t lds_doubles(const mcv::LaunchArgs& A) {
+  if (A.cache_slots <= 0) return A.lds_total;
+  return (int64_t)A.cache_off + kCacheHeaderDoubles + A.cache_slots / 2 + 3 * (int64_t)A.cache_slots;
+}
+
+void no_cache(mcv::LaunchArgs& A) {
+  A.cache_slots = 0;
+  A.cache_off = 0;
+  A.cmask = 0;
+}
+
 int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
                  mcv::LaunchArgs& args, std::vector<double>& B, std::vector<int64_t>& nbins, int64_t& hist_total,
                  std::vector<int>& types, std::vector<int>& nparams, bool replicate = false,
@@ -591,6 +618,15 @@ int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offset
     rtot += r * len;
   }
   if (rep_total) *rep_total = rtot;
+  // the LDS bin cache: every monitor not kept whole in LDS, of at most 2^20 bins
+  no_cache(args);
+  const int slots = lds_cache_slots();
+  for (int i = 0; i < ncomp && slots > 0; ++i)
+    if (packed[i].nbins > 0 && packed[i].nbins <= kCacheMaxBins && args.loff[i] < 0) args.cmask |= 1u << i;
+  if (args.cmask) {
+    args.cache_slots = slots;
+    args.cache_off = args.lds_total;
+  }
   args.ncomp = ncomp;
   return MCV_OK;
 }
@@ -833,7 +869,11 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   if (rc) return rc;
   rc = module_int(h->mod, "mcv_stream_pool", 0, &h->pool);
   if (rc) return rc;
-  kernel_info(h->fn, h->block, h->args.lds_total * sizeof(double), &h->lane_scratch, &h->per_cu);
+  int cache = 0;  // plain kernels run the LDS bin cache (and a grid of resident workgroups)
+  rc = module_int(h->mod, "mcv_lds_cache", 0, &cache);
+  if (rc) return rc;
+  if (!cache) no_cache(h->args);
+  kernel_info(h->fn, h->block, lds_doubles(h->args) * sizeof(double), &h->lane_scratch, &h->per_cu);
   return MCV_OK;
 }
 
@@ -909,7 +949,9 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
   const int64_t pool_grid =
       h->pool ? launch_grid(h->ctx, h->lane_scratch, h->block, h->per_cu, (S + per_group - 1) / per_group, true) : 0;
   if (h->pool && pool_grid <= 0) return scratch_error(h->lane_scratch, h->block, h->ctx);
-  const unsigned lds = (unsigned)(h->args.lds_total * sizeof(double));
+  const unsigned lds = (unsigned)(lds_doubles(h->args) * sizeof(double));
+  // with the bin cache, a grid of the resident workgroups, each running many stream blocks
+  const int64_t cache_grid = h->args.cache_slots > 0 ? (int64_t)h->ctx->cus * h->per_cu : 0;
   double ms_rng = 0, ms_k = 0;
   auto harvest = [&](int64_t c) -> int {
     hipEvent_t* e = h->rev + 3 * (c % kEvRing);
@@ -950,6 +992,7 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     int64_t grid = (s1 - s0 + per_group - 1) / per_group;
     if (h->pool) grid = std::min(grid, pool_grid);
+    if (cache_grid > 0) grid = std::min(grid, cache_grid);
     void* kargs[] = {&A};
     HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, (unsigned)h->block, 1, 1, lds, h->stream, kargs,
                                    nullptr));
@@ -1190,6 +1233,7 @@ int mcv_instr_adjoint(mcv_instr* h, const char* code_object, const char* kernel_
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     A.lds_nseg = 0;  // the adjoint kernel fills no histograms
     A.lds_total = 0;
+    no_cache(A);
     mcv::AdjointArgs J{h->d_G, h->d_grad, ndense > 0 ? h->d_dense : nullptr};
     const unsigned bs = (unsigned)h->adj_block;
     const int64_t per_group = (int64_t)h->adj_block * h->adj_spl;
@@ -1300,6 +1344,7 @@ int mcv_instr_process(mcv_instr* h, const char* code_object, const char* kernel_
   A.io = h->d_io;
   A.io_out = m > 0 ? h->d_io_out : nullptr;
   A.io_m = m;
+  no_cache(A);
   if (m > 0) {  // the multiple-scattering process kernel keeps no histograms
     A.lds_nseg = 0;
     A.lds_total = 0;
@@ -1522,6 +1567,7 @@ int enqueue_batch(mcv_batch* b, uint64_t seed, int64_t nthreads_max, int64_t tot
   B.base.tid_base = 0;
   B.base.tid_end = INT64_MAX;  // each instance's own stream count bounds its streams
   B.base.rng_first = 0;
+  no_cache(B.base);
   B.base.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
   B.pstride = b->pstride;
   B.hstride = b->hstride;
@@ -1685,6 +1731,7 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
     A.tid_base = 0;
     A.tid_end = nt;
     A.rng_first = 0;
+    no_cache(A);
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     void* kargs[] = {&A};
     const int64_t per_group = (int64_t)b->block * b->spl;

@@ -115,13 +115,16 @@ def resolve_shape(instrument, block=None, waves_per_eu=None, streams_per_lane=No
     """(workgroup size, occupancy target or None, streams per lane) of the
     fused kernel, the given values first, then the chain's defaults:
     config.ms_kernel_block / ms_kernel_waves_per_eu for chains with a
-    multiple-scattering sample, config.kernel_block / kernel_waves_per_eu
-    otherwise, and config.kernel_streams_per_lane."""
+    multiple-scattering sample, config.plain_kernel_block for chains that are
+    not compacted, config.kernel_block / kernel_waves_per_eu otherwise, and
+    config.kernel_streams_per_lane."""
     from . import config
     if any(c.is_multiplescattering for c in instrument.components):
         block = block or config.ms_kernel_block
         if waves_per_eu is None:
             waves_per_eu = config.ms_kernel_waves_per_eu
+    elif split_index(instrument.components) is None:
+        block = block or config.plain_kernel_block
     block = int(block or config.kernel_block)
     wpe = waves_per_eu if waves_per_eu is not None else config.kernel_waves_per_eu
     spl = int(streams_per_lane or config.kernel_streams_per_lane)

@@ -5,8 +5,12 @@ A step is one run of the workload instrument over `ncount` synthetic neutrons
 (reference partition, numba-seeded xoroshiro128+ streams), with the
 instrument's parameters and tables already resident in HBM: device-side stream
 seeding + the fused kernel + the processed-count check.  Default workload:
-the guide-transport chain (Source_simple -> 10 m guide -> 100-segment elliptic
-tapered guide -> wavelength + PSD monitors), 1e8 neutrons per step.
+the source -> guide -> sample -> monitor chain (Source_simple -> 10 m guide ->
+100-segment elliptic tapered guide -> isotropic sphere -> I(Q,E) + PSD
+monitors), 1e8 neutrons per step.  The line also carries the other configured
+chains side by side (`configs`: c1 at 1e7 neutrons in launches of 1e6, c2 with
+and without dead-neutron gating, c3 with the 400 x 128 x 500 tube array), the
+64-instance batched scan and the adjoint/forward cost.
 
 Prints one JSON line.  `roofline` prices the fused kernel against the FP64
 vector peak with the algorithmic work counted by the oracle's counting build
@@ -30,7 +34,9 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "c2": ("guide transport: Source_simple (r 5 cm, E 5-100 meV) -> 10 m straight guide 5x5 cm m=2 -> "
            "10 m 100-segment elliptic guide 6->3 cm -> wavelength (200 bins) + PSD 100x100", int(1e8)),
-    "c2s": ("guide transport + isotropic sphere sample (r 1 cm) -> I(Q,E) 120x100 + PSD 250x250", int(1e8)),
+    "c2s": ("source -> guide -> sample -> monitor: Source_simple (r 5 cm, E 5-100 meV) -> 10 m straight guide "
+            "5x5 cm m=2 -> 10 m 100-segment elliptic guide 6->3 cm -> isotropic Al sphere (r 1 cm, 5 b / 2 b) -> "
+            "I(Q,E) 120x100 + PSD 250x250", int(1e8)),
     "c1": ("Source_simple (E 60+-10 meV) -> isotropic sphere (r 1 cm, 5 b / 2 b) -> I(Q,E) 120x100 + "
            "PSD 250x250", int(1e7)),
     "c3": ("Source_simple (E 60+-10 meV) -> hollow cylinder (r 1.0/1.5 cm, h 5 cm), S(Q,E) grid 200x200, "
@@ -38,7 +44,7 @@ WORKLOADS = {
 }
 
 
-def oracle_flops_per_neutron(instr, n=1000000, seed=0):
+def oracle_flops_per_neutron(instr, n=1000000, seed=0, gate=True):
     """Mean FP64 ops per neutron over the first n neutrons (counting oracle)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -54,7 +60,7 @@ def oracle_flops_per_neutron(instr, n=1000000, seed=0):
         L.orc_flops_reset()
         descs = [c.mcv_desc() for c in instr.components]
         off, rot = calcTransformations(instr)
-        _, processed, _, _ = pyoracle.run(descs, off, rot, n, seed, gate=True, nthreads=1)
+        _, processed, _, _ = pyoracle.run(descs, off, rot, n, seed, gate=gate, nthreads=1)
         c = np.zeros(3, dtype=np.uint64)
         L.orc_flops_read(c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     finally:
@@ -64,7 +70,7 @@ def oracle_flops_per_neutron(instr, n=1000000, seed=0):
             "neutrons": int(processed)}
 
 
-def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0, cores=None):
+def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0, cores=None, gate=True):
     """Oracle on this host's cores (or `cores` threads) over a bounded leading
     part of the workload (whole streams of the reference partition)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -78,12 +84,12 @@ def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0, cores=None
     # probe, then size the sample to ~`seconds`
     probe = max(1, min(nthreads, 2000000 // max(npt, 1)))
     t = time.perf_counter()
-    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, True, (0, probe), cores)
+    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, gate, (0, probe), cores)
     dt = time.perf_counter() - t
     rate = done / dt
     streams = min(nthreads, max(probe, int(rate * seconds / max(npt, 1))))
     t = time.perf_counter()
-    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, True, (0, streams), cores)
+    _, done, _, _ = pyoracle.run(descs, off, rot, ncount, 0, ntotalthreads, tpb, gate, (0, streams), cores)
     dt = time.perf_counter() - t
     return {"value": done / dt, "unit": "neutrons/s", "cores": cores, "kind": "port",
             "sample": "first %d of %d streams of the same partition (%d neutrons, seed 0), %.1f s" % (
@@ -103,18 +109,22 @@ def batch_scan(ninst=64, reps=3):
     total = int(n.sum())
     scan, kw = tuned_scan(ins, n, use_cache=False)
     tuned = {"block": scan.block, "waves_per_eu": scan.waves_per_eu, "streams_per_lane": scan.streams_per_lane}
+    def checked(st):
+        assert st["processed"] == total, st  # (the runtime also checks every instance's count)
+        return st
+
     try:
-        scan.run(n, seed=0, **kw)
-        scan.run_sequential(n, seed=0)
-        b = [scan.run(n, seed=0, **kw)["ms_kernel"] for _ in range(reps)]
+        checked(scan.run(n, seed=0, **kw))
+        checked(scan.run_sequential(n, seed=0))
+        b = [checked(scan.run(n, seed=0, **kw))["ms_kernel"] for _ in range(reps)]
         t = time.perf_counter()
         for _ in range(reps):
-            scan.run(n, seed=0, **kw)
+            checked(scan.run(n, seed=0, **kw))
         b_wall = (time.perf_counter() - t) / reps * 1e3
-        s = [scan.run_sequential(n, seed=0)["ms_kernel"] for _ in range(reps)]
+        s = [checked(scan.run_sequential(n, seed=0))["ms_kernel"] for _ in range(reps)]
         t = time.perf_counter()
         for _ in range(reps):
-            scan.run_sequential(n, seed=0)
+            checked(scan.run_sequential(n, seed=0))
         s_wall = (time.perf_counter() - t) / reps * 1e3
     finally:
         scan.close()
@@ -130,7 +140,7 @@ def batch_scan(ninst=64, reps=3):
             run_instrument(instr, int(min(c, 1000)), seed=0)
         t = time.perf_counter()
         for instr, c in zip(singles, n):
-            run_instrument(instr, int(c), seed=0)
+            assert run_instrument(instr, int(c), seed=0)["processed"] == c
         api_wall = (time.perf_counter() - t) * 1e3
     finally:
         config.kernel_block, config.kernel_waves_per_eu = saved
@@ -188,45 +198,67 @@ def copy_kernel(nbytes=1 << 31, reps=3):
     return 0
 
 
-CONFIG_TABLE = (("c1", int(1e7)), ("c2", int(1e8)), ("c2s", int(1e8)), ("c3", int(1e8)))
+# (name, chain, neutrons per run, dead-neutron gating, neutrons per launch or 0)
+CONFIG_ROWS = (("c1", "c1", int(1e7), True, int(1e6)), ("c2", "c2", int(1e8), True, 0),
+               ("c2-ungated", "c2", int(1e8), False, 0), ("c2s", "c2s", int(1e8), True, 0),
+               ("c3", "c3", int(1e8), True, 0))
 
 
-def config_table(steps=3, cpu_seconds=5.0, falg_neutrons=100000):
-    """The benchmark configurations side by side: fused-kernel time (HIP events)
-    and wall time per run at the reference partition, neutrons/s, F_alg and
-    the FP64 fraction, and the CPU oracle on all host cores and on one.
-    c2 and c2s run at their autotuned shape, c1 and c3 at the default one."""
+def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_neutrons=1000000):
+    """The configured chains side by side: fused-kernel time (HIP events, summed
+    over a run's launches) and wall time per run at the reference partition,
+    neutrons/s, F_alg and the FP64 fraction, and the CPU oracle over a bounded
+    sample (all host cores; one core too with cpu_one_core).  c2 and c2s run
+    at their autotuned shape, c1 and c3 at the default one; c1 runs 1e7
+    neutrons in launches of 1e6 as configured; "c2-ungated" keeps absorbed
+    neutrons flowing (the reference's fused kernel exactly)."""
     from acc_amd import configs
     from acc_amd._abi import MCV_FLAG_GATE_DEAD
     from acc_amd.autotune import tuned_handle
     from acc_amd.run_script import get_handle
     rows = {}
-    for name, ncount in CONFIG_TABLE:
-        print("[configs] %s %d neutrons" % (name, ncount), file=sys.stderr, flush=True)
-        instr = configs.BUILDERS[name]()
-        if name in ("c2", "c2s"):
+    for name, chain, ncount, gate, buf in CONFIG_ROWS:
+        if names is not None and name not in names:
+            continue
+        print("[configs] %s: %d neutrons%s" % (name, ncount, " in launches of %d" % buf if buf else ""),
+              file=sys.stderr, flush=True)
+        instr = configs.BUILDERS[chain]()
+        flags = MCV_FLAG_GATE_DEAD if gate else 0
+        if chain in ("c2", "c2s"):
             h, best = tuned_handle(instr, ncount, use_cache=False)
             shape = "%s/%s/%s" % (best["block"], best["waves_per_eu"], best["streams_per_lane"])
         else:
             h, shape = get_handle(instr), "default"
         h.clear()
-        h.run(ncount, 0, 0, 0, MCV_FLAG_GATE_DEAD)
-        kms, walls = [], []
+        h.run(ncount, 0, 0, 0, flags, buffer_size=buf)
+        kms, walls, launches = [], [], 1
         for k in range(steps):
             h.clear()
             t = time.perf_counter()
-            kms.append(h.run(ncount, k, 0, 0, MCV_FLAG_GATE_DEAD)["ms_kernel"])
+            st = h.run(ncount, k, 0, 0, flags, buffer_size=buf)
             walls.append((time.perf_counter() - t) * 1e3)
-        fl = oracle_flops_per_neutron(configs.BUILDERS[name](), n=falg_neutrons)
+            assert st["processed"] == ncount, st
+            kms.append(st["ms_kernel"])
+            launches = st["nchunks"]
+        fl = oracle_flops_per_neutron(configs.BUILDERS[chain](), n=falg_neutrons, gate=gate)
         k_avg, w_avg = sum(kms) / steps, sum(walls) / steps
-        cpu_all = cpu_baseline(configs.BUILDERS[name](), ncount, seconds=cpu_seconds)
-        cpu_one = cpu_baseline(configs.BUILDERS[name](), ncount, seconds=cpu_seconds, cores=1)
-        rows[name] = {"ncount": ncount, "launch_shape": shape, "ms_kernel": k_avg, "ms_wall": w_avg,
-                      "neutrons_per_s_kernel": ncount / (k_avg * 1e-3), "neutrons_per_s_wall": ncount / (w_avg * 1e-3),
-                      "falg_ops_per_neutron": fl["F_alg"], "draws_per_neutron": fl["draws"],
-                      "fp64_frac": fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                      "cpu_all_cores": cpu_all, "cpu_one_core": cpu_one}
-    return {"configs": rows, "cpu_model": cpu_model()}
+        row = {"ncount": ncount, "gated": gate, "launches": launches, "launch_shape": shape,
+               "ms_kernel": k_avg, "ms_wall": w_avg,
+               "neutrons_per_s_kernel": ncount / (k_avg * 1e-3), "neutrons_per_s_wall": ncount / (w_avg * 1e-3),
+               "falg_ops_per_neutron": fl["F_alg"], "draws_per_neutron": fl["draws"],
+               "fp64_frac": fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+               "cpu_all_cores": cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, gate=gate)}
+        if cpu_one_core:
+            row["cpu_one_core"] = cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, cores=1,
+                                               gate=gate)
+        rows[name] = row
+    return rows
+
+
+def config_table(steps=3, cpu_seconds=5.0, falg_neutrons=100000):
+    """--configs: the rows above with one-core CPU rates as well."""
+    return {"configs": config_rows(steps=steps, cpu_seconds=cpu_seconds, cpu_one_core=True,
+                                   falg_neutrons=falg_neutrons), "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -261,13 +293,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2s", choices=sorted(WORKLOADS))
     ap.add_argument("--ncount", type=float, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--falg-neutrons", type=int, default=1000000)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuned one")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configured chains (c1, c2, c3)")
     ap.add_argument("--shape", default=None, help="fixed launch shape BLOCK/WAVES[/STREAMS_PER_LANE] (e.g. 256/3) instead of tuning")
     ap.add_argument("--copy-kernel", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration) and exit")
@@ -343,6 +376,9 @@ def main():
                    "semantics": "dead neutrons (p<=0) dropped", "launch_shape": shape},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if not a.no_configs:
+        line["configs"] = config_rows([r[0] for r in CONFIG_ROWS if r[0] != a.workload])
+        line["cpu_model"] = cpu_model()
     if not a.no_batch:
         line["batch_scan"] = batch_scan()
     if not a.no_adjoint:

@@ -213,7 +213,7 @@ def test_autograd_and_fit():
 
     opt.step(closure)
     final = float(loss_fn().detach())
-    print("[adjoint] fit loss %.3e -> %.3e, max |x| %.2e" % (first, final, float(x.abs().max())), flush=True)
+    print("[adjoint] fit loss %.3e -> %.3e, max |x| %.2e" % (first, final, float(x.detach().abs().max())), flush=True)
     for name, v in zip(params, x.detach().numpy()):
         print("[adjoint]   %-45s relative offset %.2e" % (name, v), flush=True)
     assert final < 1e-20 * first
