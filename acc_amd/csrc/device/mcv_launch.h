@@ -5,6 +5,10 @@
 
 #define MCV_MAX_COMP 32
 #define MCV_LDS_SEG 8  // monitors whose histograms a workgroup keeps in LDS
+// Bins of the LDS bin cache's key space per merge tile (a power of two).
+#define MCV_CACHE_TILE_SHIFT 12
+#define MCV_CACHE_TILE (1 << MCV_CACHE_TILE_SHIFT)
+#define MCV_CACHE_MAX_TILES 1024
 
 namespace mcv {
 
@@ -46,11 +50,21 @@ struct LaunchArgs {
   int32_t rmask[MCV_MAX_COMP];
   // LDS bin cache: monitors whose bit is set in cmask add through an
   // open-addressed table of cache_slots (a power of two) bins in dynamic LDS,
-  // starting cache_off doubles in (after the privatised histograms); claimed
-  // bins are flushed to their histogram (or replica) when the workgroup ends.
-  // cache_slots = 0: no cache.
+  // starting cache_off doubles in (after the privatised histograms).  Bin b of
+  // monitor i has key (ckbase[i] << MCV_CACHE_TILE_SHIFT) + b: the monitors'
+  // bins laid end to end in tiles of MCV_CACHE_TILE, cache_ntiles tiles in
+  // all.  When the workgroup ends it writes its claimed bins, ordered by tile,
+  // to its dump at cdump + blockIdx.x * (cache_slots / 2 + 3 * cache_slots)
+  // doubles (keys [cache_slots] int32, then N, sum p, sum p^2 [cache_slots]
+  // each) with tile t's entries at [cdoff[g][t], cdoff[g][t + 1]) of
+  // cdoff + blockIdx.x * (cache_ntiles + 1); the runtime's merge kernel adds
+  // the dumps into H.  cache_slots = 0: no cache.
   int32_t cache_slots, cache_off;
   uint32_t cmask;
+  int32_t cache_ntiles;
+  int32_t ckbase[MCV_MAX_COMP];
+  double* cdump;
+  int32_t* cdoff;
   // buffered mode: neutron i of the run is row i of io (N, 10), updated in
   // place; a multiple-scattering sample writes its io_m outputs per neutron to
   // io_out rows [i*io_m, (i+1)*io_m)

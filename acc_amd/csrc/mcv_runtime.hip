@@ -127,27 +127,55 @@ const std::vector<uint64_t>& jump_matrices() {
   return flat;
 }
 
-// state <- M state over GF(2), M one of the jump tables (128 x 128 bits).
-__device__ __forceinline__ void gf2_apply(const uint64_t* __restrict__ M, uint64_t& a, uint64_t& b) {
+// Byte tables of the jump matrices for the device: for matrix m and byte j of
+// the 128-bit state (j < 8: byte j of the first word, else byte j - 8 of the
+// second), entry x (two words) = M applied to the state whose only set bits
+// are x placed at byte j.  A product is then 16 table reads XORed together
+// instead of 128 masked column XORs.  64 KB per matrix.
+constexpr int kTabWords = 16 * 256 * 2;
+
+const std::vector<uint64_t>& jump_byte_tables() {
+  static std::vector<uint64_t> tabs;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const auto& flat = jump_matrices();
+    tabs.assign((size_t)kJumpMats * kTabWords, 0);
+    for (int m = 0; m < kJumpMats; ++m) {
+      const uint64_t* M = flat.data() + (size_t)m * 256;  // column c: M[2c], M[2c + 1]
+      uint64_t* T = tabs.data() + (size_t)m * kTabWords;
+      for (int j = 0; j < 16; ++j)
+        for (int x = 1; x < 256; ++x) {
+          const int low = __builtin_ctz((unsigned)x);
+          const int c = 8 * j + low;  // column of the lowest set bit of x
+          const size_t prev = ((size_t)j * 256 + (size_t)(x & (x - 1))) * 2, e = ((size_t)j * 256 + x) * 2;
+          T[e] = T[prev] ^ M[2 * c];
+          T[e + 1] = T[prev + 1] ^ M[2 * c + 1];
+        }
+    }
+  });
+  return tabs;
+}
+
+// state <- M state over GF(2), T the byte tables of M.
+__device__ __forceinline__ void gf2_apply(const uint64_t* __restrict__ T, uint64_t& a, uint64_t& b) {
+  const ulonglong2* __restrict__ E = reinterpret_cast<const ulonglong2*>(T);
+  ulonglong2 r[16];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = E[j * 256 + (int)((a >> (8 * j)) & 255u)];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[8 + j] = E[(8 + j) * 256 + (int)((b >> (8 * j)) & 255u)];
   uint64_t oa = 0, ob = 0;
-#pragma unroll 8
-  for (int bit = 0; bit < 64; ++bit) {
-    const uint64_t m = 0ull - ((a >> bit) & 1ull);
-    oa ^= M[2 * bit] & m;
-    ob ^= M[2 * bit + 1] & m;
-  }
-#pragma unroll 8
-  for (int bit = 0; bit < 64; ++bit) {
-    const uint64_t m = 0ull - ((b >> bit) & 1ull);
-    oa ^= M[128 + 2 * bit] & m;
-    ob ^= M[128 + 2 * bit + 1] & m;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    oa ^= r[j].x;
+    ob ^= r[j].y;
   }
   a = oa;
   b = ob;
 }
 
 // Stream first + i's state is J^i applied to stream `first`'s state (a0, b0)
-// (J: the 2^64-step jump, mats[k] = J^(2^k)).  Thread t computes stream
+// (J: the 2^64-step jump; mats + k * kTabWords: the byte tables of J^(2^k)).  Thread t computes stream
 // first + t from the set bits of t, then streams t + 2^k, t + 2 * 2^k, ...
 // (kSeedChunk of them at most), each one product with J^(2^k) after the
 // previous: most streams cost one matrix product instead of one per set bit
@@ -167,11 +195,11 @@ __global__ void __launch_bounds__(256) mcv_seed_streams(const uint64_t* __restri
   if (t >= step || t >= n) return;
   uint64_t a = a0, b = b0;
   for (int bit = 0; bit < k; ++bit)
-    if ((t >> bit) & 1) gf2_apply(mats + (size_t)bit * 256, a, b);
+    if ((t >> bit) & 1) gf2_apply(mats + (size_t)bit * kTabWords, a, b);
   for (int64_t i = t; i < n; i += step) {
     out[2 * i] = a;
     out[2 * i + 1] = b;
-    if (i + step < n) gf2_apply(mats + (size_t)k * 256, a, b);
+    if (i + step < n) gf2_apply(mats + (size_t)k * kTabWords, a, b);
   }
 }
 
@@ -230,9 +258,9 @@ int device_ctx(DeviceCtx** out) {
     }
   DeviceCtx c;
   c.device = dev;
-  const auto& flat = jump_matrices();
-  HIPCHECK(hipMalloc(&c.d_mats, flat.size() * sizeof(uint64_t)));
-  HIPCHECK(hipMemcpy(c.d_mats, flat.data(), flat.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+  const auto& tabs = jump_byte_tables();
+  HIPCHECK(hipMalloc(&c.d_mats, tabs.size() * sizeof(uint64_t)));
+  HIPCHECK(hipMemcpy(c.d_mats, tabs.data(), tabs.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
   HIPCHECK(hipStreamCreate(&c.stream));
   HIPCHECK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
   c.scratch_budget = default_scratch_budget();
@@ -524,29 +552,159 @@ constexpr int64_t kReplicaBytes = int64_t(256) << 20;
 
 // Slots of the LDS bin cache of plain kernels (MCV_LDS_CACHE_SLOTS from the
 // environment, rounded down to a power of two, at least 64; 0 turns it off),
-// default 2048 (57 KB of LDS per workgroup).  Monitors of at most 2^20 bins
-// that are not privatised whole use it.
+// default 4096 (113 KB of LDS per workgroup: one workgroup per CU).  Monitors
+// of at most 2^20 bins that are not privatised whole use it.
 int lds_cache_slots() {
   const char* e = std::getenv("MCV_LDS_CACHE_SLOTS");
-  long long v = (e && *e) ? std::atoll(e) : 2048;
+  long long v = (e && *e) ? std::atoll(e) : 4096;
   if (v < 64) return 0;
   int s = 64;
   while ((long long)s * 2 <= std::min<long long>(v, 8192)) s *= 2;
   return s;
 }
 constexpr int64_t kCacheMaxBins = int64_t(1) << 20;
-constexpr int kCacheHeaderDoubles = 2 * MCV_MAX_COMP;
+constexpr int64_t kLdsBytes = 160 * 1024;  // LDS of a gfx950 CU
 
-// Dynamic LDS of a launch, in doubles: privatised histograms, then the bin cache.
+// Dynamic LDS of a launch, in doubles: privatised histograms, then the bin
+// cache (keys, three value rows, tile counters).
 int64_t lds_doubles(const mcv::LaunchArgs& A) {
   if (A.cache_slots <= 0) return A.lds_total;
-  return (int64_t)A.cache_off + kCacheHeaderDoubles + A.cache_slots / 2 + 3 * (int64_t)A.cache_slots;
+  return (int64_t)A.cache_off + A.cache_slots / 2 + 3 * (int64_t)A.cache_slots + (A.cache_ntiles + 2) / 2;
+}
+
+// Doubles of one workgroup's cache dump (keys, then N, sum p, sum p^2).
+int64_t cache_dump_doubles(const mcv::LaunchArgs& A) {
+  return A.cache_slots / 2 + 3 * (int64_t)A.cache_slots;
 }
 
 void no_cache(mcv::LaunchArgs& A) {
   A.cache_slots = 0;
   A.cache_off = 0;
   A.cmask = 0;
+  A.cache_ntiles = 0;
+  for (int i = 0; i < MCV_MAX_COMP; ++i) A.ckbase[i] = 0;
+  A.cdump = nullptr;
+  A.cdoff = nullptr;
+}
+
+int cache_tiles(int64_t nbins) { return (int)((nbins + MCV_CACHE_TILE - 1) >> MCV_CACHE_TILE_SHIFT); }
+
+// Monitors of the bin cache, for the merge kernel: tile t of the dumps belongs
+// to monitor comp[k] for base[k] <= t < base[k + 1], histogram at H + hoff[k].
+struct CacheMergeArgs {
+  double* H;
+  const double* dump;
+  const int32_t* doff;
+  int32_t groups, slots, ntiles, nmon;
+  int32_t split;  // workgroups per tile, each taking kMergeGroups dumps
+  int32_t comp_base[MCV_MAX_COMP + 1];
+  int64_t hoff[MCV_MAX_COMP], nbins[MCV_MAX_COMP];
+};
+constexpr int kMergeGroups = 8;   // dumps per merge workgroup (<= 64)
+constexpr int kMergeBatch = 4;    // entries a thread loads before adding them
+
+// kMergeGroups dumps per workgroup, `split` workgroups per tile of the cache's
+// key space: a workgroup lays its dumps' runs for its tile end to end (a
+// prefix sum over at most 64 runs), adds every entry into an LDS copy of the
+// tile (LDS FP64 atomics, kMergeBatch independent loads per thread in
+// flight), then the tile's non-empty bins into the histogram: with plain
+// read-modify-writes when it is the tile's only workgroup, else with
+// device-scope atomics (the fused kernel's direct adds are complete: the
+// stream orders the launches).
+__global__ __launch_bounds__(1024) void mcv_cache_merge(CacheMergeArgs M) {
+  __shared__ double tile[3 * MCV_CACHE_TILE];
+  __shared__ int run_lo[64], run_pre[65];
+  const int t = (int)blockIdx.x / M.split, part = (int)blockIdx.x % M.split;
+  const int g0 = part * kMergeGroups;
+  int k = 0;
+  while (k + 1 < M.nmon && M.comp_base[k + 1] <= t) ++k;
+  for (int j = threadIdx.x; j < 3 * MCV_CACHE_TILE; j += blockDim.x) tile[j] = 0;
+  if (threadIdx.x < 64) {
+    const int lane = (int)threadIdx.x, g = g0 + lane;
+    int lo = 0, n = 0;
+    if (lane < kMergeGroups && g < M.groups) {
+      const int32_t* off = M.doff + (int64_t)g * (M.ntiles + 1);
+      lo = off[t];
+      n = off[t + 1] - lo;
+    }
+    int incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    run_lo[lane] = lo;
+    run_pre[lane + 1] = incl;
+    if (lane == 0) run_pre[0] = 0;
+  }
+  __syncthreads();
+  const int total = run_pre[64];
+  const int64_t stride = M.slots / 2 + 3 * (int64_t)M.slots;
+  for (int i0 = 0; i0 < total; i0 += kMergeBatch * (int)blockDim.x) {
+    int bin[kMergeBatch];
+    double v0[kMergeBatch], v1[kMergeBatch], v2[kMergeBatch];
+#pragma unroll
+    for (int u = 0; u < kMergeBatch; ++u) {
+      const int i = i0 + u * (int)blockDim.x + (int)threadIdx.x;
+      bin[u] = -1;
+      if (i < total) {
+        int lo = 0, hi = 64;  // run_pre[lo] <= i < run_pre[hi]: the last run starting at or before i
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (run_pre[mid] <= i) lo = mid; else hi = mid;
+        }
+        const int64_t e = (int64_t)(g0 + lo) * stride;
+        const int j = run_lo[lo] + (i - run_pre[lo]);
+        bin[u] = reinterpret_cast<const int32_t*>(M.dump + e)[j] & (MCV_CACHE_TILE - 1);
+        const double* dval = M.dump + e + M.slots / 2;
+        v0[u] = dval[j];
+        v1[u] = dval[M.slots + j];
+        v2[u] = dval[2 * M.slots + j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kMergeBatch; ++u) {
+      if (bin[u] < 0) continue;
+      atomicAdd(tile + bin[u], v0[u]);
+      atomicAdd(tile + MCV_CACHE_TILE + bin[u], v1[u]);
+      atomicAdd(tile + 2 * MCV_CACHE_TILE + bin[u], v2[u]);
+    }
+  }
+  __syncthreads();
+  const int64_t nb = M.nbins[k];
+  const int64_t bin0 = (int64_t)(t - M.comp_base[k]) << MCV_CACHE_TILE_SHIFT;
+  double* H = M.H + M.hoff[k];
+  for (int j = threadIdx.x; j < MCV_CACHE_TILE; j += blockDim.x) {
+    const int64_t b = bin0 + j;
+    if (b >= nb || tile[j] == 0) continue;
+    if (M.split == 1) {
+      H[b] += tile[j];
+      H[nb + b] += tile[MCV_CACHE_TILE + j];
+      H[2 * nb + b] += tile[2 * MCV_CACHE_TILE + j];
+    } else {
+      unsafeAtomicAdd(H + b, tile[j]);
+      unsafeAtomicAdd(H + nb + b, tile[MCV_CACHE_TILE + j]);
+      unsafeAtomicAdd(H + 2 * nb + b, tile[2 * MCV_CACHE_TILE + j]);
+    }
+  }
+}
+
+// Monitors behind the bin cache take few global adds (the cache's misses and
+// one flush per workgroup), so they are not replicated: drops their replicas
+// and packs the others' offsets again.  Returns the replica doubles left.
+int64_t unreplicate_cached(mcv::LaunchArgs& A, const std::vector<int64_t>& nbins) {
+  int64_t rtot = 0;
+  for (int i = 0; i < A.ncomp; ++i) {
+    if (A.roff[i] < 0) continue;
+    if ((A.cmask >> i) & 1u) {
+      A.roff[i] = -1;
+      A.rmask[i] = 0;
+      continue;
+    }
+    A.roff[i] = rtot;
+    rtot += (int64_t)(A.rmask[i] + 1) * 3 * nbins[i];
+  }
+  return rtot;
 }
 
 int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets, const double* rotmats,
@@ -618,14 +776,21 @@ int build_layout(const mcv_comp_desc* comps, int32_t ncomp, const double* offset
     rtot += r * len;
   }
   if (rep_total) *rep_total = rtot;
-  // the LDS bin cache: every monitor not kept whole in LDS, of at most 2^20 bins
+  // the LDS bin cache: the monitors not kept whole in LDS, of at most 2^20
+  // bins, in chain order while their tiles number at most MCV_CACHE_MAX_TILES
   no_cache(args);
   const int slots = lds_cache_slots();
-  for (int i = 0; i < ncomp && slots > 0; ++i)
-    if (packed[i].nbins > 0 && packed[i].nbins <= kCacheMaxBins && args.loff[i] < 0) args.cmask |= 1u << i;
+  for (int i = 0; i < ncomp && slots > 0; ++i) {
+    if (packed[i].nbins <= 0 || packed[i].nbins > kCacheMaxBins || args.loff[i] >= 0) continue;
+    if (args.cache_ntiles + cache_tiles(packed[i].nbins) > MCV_CACHE_MAX_TILES) continue;
+    args.cmask |= 1u << i;
+    args.ckbase[i] = args.cache_ntiles;
+    args.cache_ntiles += cache_tiles(packed[i].nbins);
+  }
   if (args.cmask) {
     args.cache_slots = slots;
     args.cache_off = args.lds_total;
+    while (args.cache_slots > 64 && lds_doubles(args) * (int64_t)sizeof(double) > kLdsBytes) args.cache_slots >>= 1;
   }
   args.ncomp = ncomp;
   return MCV_OK;
@@ -676,6 +841,8 @@ struct mcv_instr {
   double* d_hist = nullptr;
   double* d_rep = nullptr;  // monitor replicas (zero between launches)
   int64_t rep_total = 0;
+  double* d_cdump = nullptr;  // bin-cache dumps of the resident workgroups
+  int32_t* d_cdoff = nullptr;
   uint64_t* d_rng = nullptr;
   int64_t rng_cap = 0;
   unsigned long long* d_counter = nullptr;
@@ -805,6 +972,8 @@ void release(mcv_instr* h) {
   if (h->d_blob) (void)hipFree(h->d_blob);
   if (h->d_hist) (void)hipFree(h->d_hist);
   if (h->d_rep) (void)hipFree(h->d_rep);
+  if (h->d_cdump) (void)hipFree(h->d_cdump);
+  if (h->d_cdoff) (void)hipFree(h->d_cdoff);
   if (h->d_rng) (void)hipFree(h->d_rng);
   if (h->d_counter) (void)hipFree(h->d_counter);
   if (h->d_dump) (void)hipFree(h->d_dump);
@@ -854,10 +1023,6 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   HIPCHECK(hipMemcpy(h->d_blob, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMalloc(&h->d_hist, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
   HIPCHECK(hipMemset(h->d_hist, 0, std::max<int64_t>(h->hist_total, 1) * sizeof(double)));
-  if (h->rep_total > 0) {
-    HIPCHECK(hipMalloc(&h->d_rep, h->rep_total * sizeof(double)));
-    HIPCHECK(hipMemset(h->d_rep, 0, h->rep_total * sizeof(double)));
-  }
   HIPCHECK(hipMalloc(&h->d_counter, 2 * sizeof(unsigned long long)));  // processed, stream pool
 
   hipError_t e = hipModuleLoad(&h->mod, code_object);
@@ -873,7 +1038,43 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   rc = module_int(h->mod, "mcv_lds_cache", 0, &cache);
   if (rc) return rc;
   if (!cache) no_cache(h->args);
+  h->rep_total = unreplicate_cached(h->args, h->nbins);
+  if (h->rep_total > 0) {
+    HIPCHECK(hipMalloc(&h->d_rep, h->rep_total * sizeof(double)));
+    HIPCHECK(hipMemset(h->d_rep, 0, h->rep_total * sizeof(double)));
+  }
   kernel_info(h->fn, h->block, lds_doubles(h->args) * sizeof(double), &h->lane_scratch, &h->per_cu);
+  if (h->args.cache_slots > 0) {  // one dump per resident workgroup (the cache's launch grid)
+    const int64_t groups = (int64_t)h->ctx->cus * std::max(h->per_cu, 1);
+    HIPCHECK(hipMalloc(&h->d_cdump, (size_t)(groups * cache_dump_doubles(h->args)) * sizeof(double)));
+    HIPCHECK(hipMalloc(&h->d_cdoff, (size_t)(groups * (h->args.cache_ntiles + 1)) * sizeof(int32_t)));
+    h->args.cdump = h->d_cdump;
+    h->args.cdoff = h->d_cdoff;
+  }
+  return MCV_OK;
+}
+
+// Adds the bin-cache dumps of a launch's `groups` workgroups into the histograms.
+int merge_cache(mcv_instr* h, int64_t groups) {
+  const mcv::LaunchArgs& A = h->args;
+  CacheMergeArgs M{};
+  M.H = h->d_hist;
+  M.dump = h->d_cdump;
+  M.doff = h->d_cdoff;
+  M.groups = (int32_t)groups;
+  M.slots = A.cache_slots;
+  M.ntiles = A.cache_ntiles;
+  for (int i = 0; i < h->ncomp; ++i) {
+    if (!((A.cmask >> i) & 1u)) continue;
+    M.comp_base[M.nmon] = A.ckbase[i];
+    M.hoff[M.nmon] = A.hoff[i];
+    M.nbins[M.nmon] = h->nbins[i];
+    M.nmon += 1;
+  }
+  M.comp_base[M.nmon] = A.cache_ntiles;
+  M.split = (int32_t)std::max<int64_t>(1, (groups + kMergeGroups - 1) / kMergeGroups);
+  hipLaunchKernelGGL(mcv_cache_merge, dim3((unsigned)(A.cache_ntiles * M.split)), dim3(1024), 0, h->stream, M);
+  HIPCHECK(hipGetLastError());
   return MCV_OK;
 }
 
@@ -951,7 +1152,7 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
   if (h->pool && pool_grid <= 0) return scratch_error(h->lane_scratch, h->block, h->ctx);
   const unsigned lds = (unsigned)(lds_doubles(h->args) * sizeof(double));
   // with the bin cache, a grid of the resident workgroups, each running many stream blocks
-  const int64_t cache_grid = h->args.cache_slots > 0 ? (int64_t)h->ctx->cus * h->per_cu : 0;
+  const int64_t cache_grid = h->args.cache_slots > 0 ? (int64_t)h->ctx->cus * std::max(h->per_cu, 1) : 0;
   double ms_rng = 0, ms_k = 0;
   auto harvest = [&](int64_t c) -> int {
     hipEvent_t* e = h->rev + 3 * (c % kEvRing);
@@ -996,6 +1197,7 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
     void* kargs[] = {&A};
     HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, (unsigned)h->block, 1, 1, lds, h->stream, kargs,
                                    nullptr));
+    if (cache_grid > 0 && (rc = merge_cache(h, grid))) return rc;  // before the next launch reuses the dumps
     if (c + 1 == nchunks && (rc = fold_replicas(h))) return rc;
     HIPCHECK(hipEventRecord(e[2], h->stream));
   }

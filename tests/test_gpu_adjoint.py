@@ -217,4 +217,4 @@ def test_autograd_and_fit():
     for name, v in zip(params, x.detach().numpy()):
         print("[adjoint]   %-45s relative offset %.2e" % (name, v), flush=True)
     assert final < 1e-20 * first
-    assert float(x.abs().max()) < 1e-8
+    assert float(x.detach().abs().max()) < 1e-8

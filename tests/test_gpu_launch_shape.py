@@ -93,7 +93,8 @@ def test_ms_compaction_invariance(name):
 def test_replicated_histograms(name, monkeypatch):
     """Monitors whose adds are spread over replicas (and folded after the
     launch) give the histograms of unreplicated adds, on every run of a
-    handle (the fold leaves the replicas zeroed for the next one)."""
+    handle (the fold leaves the replicas zeroed for the next one).  The LDS
+    bin cache is off: monitors behind it are not replicated."""
     from acc_amd._abi import Handle
     from acc_amd.fuser import fuse
     from acc_amd.instrument import calcTransformations
@@ -103,6 +104,7 @@ def test_replicated_histograms(name, monkeypatch):
     p, k = fuse(instr)
     N, ntot = 400000, 20000
     res = {}
+    monkeypatch.setenv("MCV_LDS_CACHE_SLOTS", "0")
     for reps in ("1", "64"):
         monkeypatch.setenv("MCV_HIST_REPLICAS", reps)
         h = Handle([c.mcv_desc() for c in instr.components], off, rot, p, k)
