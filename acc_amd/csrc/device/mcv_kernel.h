@@ -29,6 +29,11 @@
 #if MCV_BLOCK % 64 != 0 || MCV_BLOCK < 64 || MCV_BLOCK > 1024
 #error "MCV_BLOCK must be a multiple of 64 in [64, 1024]"
 #endif
+// The LDS bin cache is compiled into the plain kernels only (the fuser
+// defines MCV_LDS_CACHE 1 for them); elsewhere its code would cost registers.
+#ifndef MCV_LDS_CACHE
+#define MCV_LDS_CACHE 0
+#endif
 #define MCV_QCAP (2 * MCV_BLOCK)  // >= (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
 
 // Random streams per lane (1 except in stream-carrying compacted kernels): a
@@ -143,7 +148,7 @@ __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_
     atomicAdd(L + b, 1.0);
     atomicAdd(L + nbins + b, p);
     atomicAdd(L + 2 * nbins + b, p * p);
-  } else if (((A.cmask >> comp) & 1u) && cache_add(A, comp, b, p)) {
+  } else if (MCV_LDS_CACHE && ((A.cmask >> comp) & 1u) && cache_add(A, comp, b, p)) {
     return;
   } else {
     hist_add3(hist_block(A, comp, nbins), nbins, b, p);
@@ -152,9 +157,10 @@ __device__ __forceinline__ void hist_count(const LaunchArgs& A, int comp, int64_
 
 // All threads of the workgroup call these (barriers when LDS histograms or the cache are in use).
 __device__ __forceinline__ void lds_hist_zero(const LaunchArgs& A) {
-  if (A.lds_nseg == 0 && A.cache_slots == 0) return;
+  const bool cache = MCV_LDS_CACHE && A.cache_slots > 0;
+  if (A.lds_nseg == 0 && !cache) return;
   for (int j = threadIdx.x; j < A.lds_total; j += blockDim.x) mcv_lds_hist[j] = 0;
-  if (A.cache_slots > 0) {
+  if (cache) {
     double* base = mcv_lds_hist + A.cache_off;
     int32_t* keys = reinterpret_cast<int32_t*>(base);
     double* val = base + (A.cache_slots >> 1);
@@ -215,9 +221,10 @@ __device__ __forceinline__ void cache_dump(const LaunchArgs& A) {
 }
 
 __device__ __forceinline__ void lds_hist_flush(const LaunchArgs& A) {
-  if (A.lds_nseg == 0 && A.cache_slots == 0) return;
+  const bool cache = MCV_LDS_CACHE && A.cache_slots > 0;
+  if (A.lds_nseg == 0 && !cache) return;
   __syncthreads();
-  if (A.cache_slots > 0) cache_dump(A);
+  if (cache) cache_dump(A);
   // constant indices only: the batch kernel keeps its LaunchArgs copy out of
   // scratch memory only if no field is indexed at run time
 #pragma unroll
@@ -704,7 +711,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 }  // namespace mcv
 
 #define MCV_DEFINE_KERNELS_PLAIN(NAME, CHAIN)                                                  \
-  extern "C" __device__ int mcv_lds_cache = 1;                                                \
+  extern "C" __device__ int mcv_lds_cache = MCV_LDS_CACHE;                                    \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
     mcv::run_lanes<CHAIN>(A);                                                                 \
   }                                                                                           \

@@ -149,6 +149,8 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
             raise ValueError("waves per SIMD must be in [1, 8], got %r" % wpe)
         defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
     src, structure = _generate(instrument, compact)
+    if "MCV_DEFINE_KERNELS_PLAIN(" in src:
+        defs += "#define MCV_LDS_CACHE 1\n"
     if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src:
         if not 1 <= spl <= 16:
             raise ValueError("streams per lane must be in [1, 16], got %d" % spl)
