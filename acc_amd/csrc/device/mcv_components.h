@@ -569,20 +569,67 @@ __device__ __forceinline__ int64_t psd4pi_bin(Neutron& n, const double* __restri
 }
 
 // acc/components/monitors/iqe_monitor.py:80-113.
-// P = Ei, Qmin, Qmax, Emin, Emax, max_in, min_in, max_out, min_out, nQ, nE
-__device__ __forceinline__ int64_t iqe_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
-  const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
-  nbins = nQ * nE;
-  const double vx = n.vx, vy = n.vy, vz = n.vz;
+// P = Ei, Qmin, Qmax, Emin, Emax, max_in, min_in, max_out, min_out, nQ, nE,
+// then the angular window in the form iqe_window reads (derived by the
+// runtime): [11], [12] the pseudo-angles of min_in and max_in (-3 / 3 for
+// bounds no direction reaches), [13] kind of the min_out test (0 never, 1
+// always, 2 compare), [14], [15] cos and sin of min_out, [16]..[18] the same
+// for max_out.
+
+// The reference's window test: in-plane angle atan2(vx, vz) and out-of-plane
+// angle 90 -+ atan(sqrt(vx^2 + vz^2) / vy) in degrees, strict bounds.  Kept
+// out of line: iqe_window calls it only for directions within 1e-12 of a
+// window edge, and its atan2 / atan would otherwise raise the register
+// pressure of every fused kernel holding an I(Q,E) monitor.
+__device__ __noinline__ bool iqe_window_exact(double vx, double vy, double vz, const double* __restrict__ P) {
   const double angle_in_plane = atan2(vx, vz) / PI * 180.;
   double angle_out_of_plane = 0.;
   if (vy != 0) {
     const double theta = atan(sqrt(vx * vx + vz * vz) / vy) / PI * 180.;
     angle_out_of_plane = vy > 0 ? 90. - theta : 90. + theta;
   }
-  if (!(P[6] < angle_in_plane && angle_in_plane < P[5] && P[8] < angle_out_of_plane &&
-        angle_out_of_plane < P[7]))
-    return -1;
+  return P[6] < angle_in_plane && angle_in_plane < P[5] && P[8] < angle_out_of_plane && angle_out_of_plane < P[7];
+}
+
+// Pseudo-angle of the in-plane direction (vz, vx): vx / (|vx| + |vz|) for vz >
+// 0, 2 - that for vx > 0 >= vz, -2 - that for vx < 0 >= vz; strictly
+// increasing in atan2(vx, vz) over (-pi, pi), from -2 to 2, with slope at least
+// 1/2 per radian.
+__device__ __forceinline__ double iqe_pseudo_angle(double vx, double vz, double s) {
+  const double f = vx / s;
+  return vz > 0 ? f : (vx > 0 ? 2.0 - f : -2.0 - f);
+}
+
+// The same decision without atan2 / atan.  The in-plane bounds compare
+// pseudo-angles; the out-of-plane angle is the elevation el = atan(|vy| / r),
+// r = sqrt(vx^2 + vz^2), in [0, 90] degrees (the reference's 90 -+ theta for
+// either sign of vy), and el > b is the sign of |vy| cos b - r sin b.  Every
+// quantity is within ~1e-15 (relative to its scale) of its exact value, and
+// the reference's angles within ~1e-15 rad of the true ones, so a direction
+// more than 1e-12 from every bound gets the reference's answer from the signs;
+// one closer (or with vx = 0, non-finite components or bounds) takes
+// iqe_window_exact.
+__device__ __forceinline__ bool iqe_window(double vx, double vy, double vz, const double* __restrict__ P) {
+  constexpr double kMargin = 1e-12;
+  const double s = fabs(vx) + fabs(vz), ay = fabs(vy);
+  const double p = iqe_pseudo_angle(vx, vz, s);
+  const double r = sqrt(vx * vx + vz * vz), w = ay + r;
+  const double qlo = ay * P[14] - r * P[15], qhi = ay * P[17] - r * P[18];
+  const bool tlo = P[13] == 2.0, thi = P[16] == 2.0;
+  const bool sure = vx != 0 && s < INFINITY && ay < INFINITY && fabs(p - P[11]) > kMargin &&
+                    fabs(p - P[12]) > kMargin && (!tlo || fabs(qlo) > kMargin * w) &&
+                    (!thi || fabs(qhi) > kMargin * w);
+  if (!sure) return iqe_window_exact(vx, vy, vz, P);
+  const bool above = tlo ? qlo > 0 : P[13] == 1.0;  // el > min_out
+  const bool below = thi ? qhi < 0 : P[16] == 1.0;  // el < max_out
+  return p > P[11] && p < P[12] && above && below;
+}
+
+__device__ __forceinline__ int64_t iqe_bin(const Neutron& n, const double* __restrict__ P, int64_t& nbins) {
+  const int64_t nQ = (int64_t)P[9], nE = (int64_t)P[10];
+  nbins = nQ * nE;
+  const double vx = n.vx, vy = n.vy, vz = n.vz;
+  if (!iqe_window(vx, vy, vz, P)) return -1;
   const double Ei = P[0], Qmin = P[1], Qmax = P[2], Emin = P[3], Emax = P[4];
   const double vi = sqrt(Ei) * SE2V;
   const double Q = sqrt(vx * vx + vy * vy + (vz - vi) * (vz - vi)) * V2K;

@@ -379,6 +379,42 @@ void taper_records(double l_seg, int64_t nseg, const double* T, double* D) {  //
                  T[7 * nseg + k], D + kTaperRec * k);
 }
 
+// I(Q,E) monitor's angular window in the form the device test reads
+// (mcv_components.h iqe_window), from P = Ei, Qmin, Qmax, Emin, Emax, max_in,
+// min_in, max_out, min_out (degrees; acc/components/monitors/iqe_monitor.py:
+// 93-101): out[0], out[1] = pseudo-angles of min_in and max_in, out[2..4] =
+// kind (0 never, 1 always, 2 compare), cos, sin of the min_out test "el >
+// min_out", out[5..7] the same for "el < max_out" (el in [0, 90]).
+double iqe_pseudo_angle_host(double deg) {
+  const double phi = deg * (M_PI / 180.0);
+  const double vz = std::cos(phi), vx = std::sin(phi), s = std::fabs(vx) + std::fabs(vz);
+  const double f = vx / s;
+  return vz > 0 ? f : (vx > 0 ? 2.0 - f : -2.0 - f);
+}
+
+void iqe_window_params(const double* P, double* out) {
+  const double max_in = P[5], min_in = P[6], max_out = P[7], min_out = P[8];
+  // angle_in_plane lies in [-180, 180]: a > min_in always below -180, never from 180 on
+  if (std::isnan(min_in)) out[0] = NAN;
+  else if (min_in < -180.0) out[0] = -3.0;
+  else if (min_in == -180.0) out[0] = -2.0;
+  else if (min_in >= 180.0) out[0] = 3.0;
+  else out[0] = iqe_pseudo_angle_host(min_in);
+  if (std::isnan(max_in)) out[1] = NAN;
+  else if (max_in > 180.0) out[1] = 3.0;
+  else if (max_in == 180.0) out[1] = 2.0;
+  else if (max_in <= -180.0) out[1] = -3.0;
+  else out[1] = iqe_pseudo_angle_host(max_in);
+  // el > min_out: always below 0, never from 90 on; el < max_out: never up to 0, always above 90
+  const double lo = min_out * (M_PI / 180.0), hi = max_out * (M_PI / 180.0);
+  out[2] = std::isnan(min_out) ? 0.0 : (min_out < 0.0 ? 1.0 : (min_out >= 90.0 ? 0.0 : 2.0));
+  out[3] = std::cos(lo);
+  out[4] = std::sin(lo);
+  out[5] = std::isnan(max_out) ? 0.0 : (max_out <= 0.0 ? 0.0 : (max_out > 90.0 ? 1.0 : 2.0));
+  out[6] = std::cos(hi);
+  out[7] = std::sin(hi);
+}
+
 void cross3(const double* a, const double* b, double* o) {
   o[0] = a[1] * b[2] - a[2] * b[1];
   o[1] = a[2] * b[0] - a[0] * b[2];
@@ -505,6 +541,8 @@ int pack(const mcv_comp_desc& d, int i, Packed& out) {
       copyp(9);
       out.params.push_back((double)d.iparams[0]);
       out.params.push_back((double)d.iparams[1]);
+      out.params.resize(19);
+      iqe_window_params(out.params.data(), out.params.data() + 11);
       out.nbins = d.n_bins;
       return MCV_OK;
     }
@@ -1268,6 +1306,10 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params, int32
       taper_record(params[0], k, r[0], r[12], r[2], r[3], r[13], r[1], r);
     }
     HIPCHECK(hipMemcpy(h->d_blob + h->args.doff[comp], D, sizeof(double) * kTaperRec * nseg, hipMemcpyHostToDevice));
+  } else if (h->types[comp] == MCV_MON_IQE) {
+    double* P = h->blob.data() + off;
+    iqe_window_params(P, P + 11);
+    HIPCHECK(hipMemcpy(h->d_blob + off + 11, P + 11, sizeof(double) * 8, hipMemcpyHostToDevice));
   }
   return MCV_OK;
 }
