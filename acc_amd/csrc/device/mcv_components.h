@@ -580,8 +580,10 @@ __device__ __forceinline__ int64_t psd4pi_bin(Neutron& n, const double* __restri
 // angle 90 -+ atan(sqrt(vx^2 + vz^2) / vy) in degrees, strict bounds.  Kept
 // out of line: iqe_window calls it only for directions within 1e-12 of a
 // window edge, and its atan2 / atan would otherwise raise the register
-// pressure of every fused kernel holding an I(Q,E) monitor.
-__device__ __noinline__ bool iqe_window_exact(double vx, double vy, double vz, const double* __restrict__ P) {
+// pressure of every fused kernel holding an I(Q,E) monitor (marked cold, so
+// the register allocator keeps the saves around the call on that rare path:
+// c2s 24.6 ms with the angles inline, 24.2 as a plain call, 23.8 cold).
+__device__ __noinline__ __attribute__((cold)) bool iqe_window_exact(double vx, double vy, double vz, const double* __restrict__ P) {
   const double angle_in_plane = atan2(vx, vz) / PI * 180.;
   double angle_out_of_plane = 0.;
   if (vy != 0) {
@@ -619,7 +621,7 @@ __device__ __forceinline__ bool iqe_window(double vx, double vy, double vz, cons
   const bool sure = vx != 0 && s < INFINITY && ay < INFINITY && fabs(p - P[11]) > kMargin &&
                     fabs(p - P[12]) > kMargin && (!tlo || fabs(qlo) > kMargin * w) &&
                     (!thi || fabs(qhi) > kMargin * w);
-  if (!sure) return iqe_window_exact(vx, vy, vz, P);
+  if (__builtin_expect(!sure, 0)) return iqe_window_exact(vx, vy, vz, P);
   const bool above = tlo ? qlo > 0 : P[13] == 1.0;  // el > min_out
   const bool below = thi ? qhi < 0 : P[16] == 1.0;  // el < max_out
   return p > P[11] && p < P[12] && above && below;

@@ -198,8 +198,11 @@ def copy_kernel(nbytes=1 << 31, reps=3):
     return 0
 
 
-# (name, chain, neutrons per run, dead-neutron gating, neutrons per launch or 0)
-CONFIG_ROWS = (("c1", "c1", int(1e7), True, int(1e6)), ("c2", "c2", int(1e8), True, 0),
+# (name, chain, neutrons per run, dead-neutron gating, neutrons per launch or 0).  c1 is
+# configured in buffers of 1e6 neutrons; the reference's runner ignores buffer_size
+# (acc/run_script.py:316-328) and runs it as one launch, which "c1-one-launch" times.
+CONFIG_ROWS = (("c1", "c1", int(1e7), True, int(1e6)), ("c1-one-launch", "c1", int(1e7), True, 0),
+               ("c2", "c2", int(1e8), True, 0),
                ("c2-ungated", "c2", int(1e8), False, 0), ("c2s", "c2s", int(1e8), True, 0),
                ("c3", "c3", int(1e8), True, 0))
 
@@ -210,13 +213,14 @@ def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_n
     neutrons/s, F_alg and the FP64 fraction, and the CPU oracle over a bounded
     sample (all host cores; one core too with cpu_one_core).  c2 and c2s run
     at their autotuned shape, c1 and c3 at the default one; c1 runs 1e7
-    neutrons in launches of 1e6 as configured; "c2-ungated" keeps absorbed
-    neutrons flowing (the reference's fused kernel exactly)."""
+    neutrons in launches of 1e6 as configured and "c1-one-launch" in one
+    launch; "c2-ungated" keeps absorbed neutrons flowing (the reference's fused
+    kernel exactly).  Rows of the same chain share its F_alg and CPU figures."""
     from acc_amd import configs
     from acc_amd._abi import MCV_FLAG_GATE_DEAD
     from acc_amd.autotune import tuned_handle
     from acc_amd.run_script import get_handle
-    rows = {}
+    rows, measured = {}, {}  # measured: (chain, gate) -> F_alg and CPU figures of an earlier row
     for name, chain, ncount, gate, buf in CONFIG_ROWS:
         if names is not None and name not in names:
             continue
@@ -240,17 +244,23 @@ def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_n
             assert st["processed"] == ncount, st
             kms.append(st["ms_kernel"])
             launches = st["nchunks"]
-        fl = oracle_flops_per_neutron(configs.BUILDERS[chain](), n=falg_neutrons, gate=gate)
+        if (chain, gate) not in measured:
+            m = {"fl": oracle_flops_per_neutron(configs.BUILDERS[chain](), n=falg_neutrons, gate=gate),
+                 "cpu": cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, gate=gate)}
+            if cpu_one_core:
+                m["cpu1"] = cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, cores=1, gate=gate)
+            measured[(chain, gate)] = m
+        m = measured[(chain, gate)]
+        fl = m["fl"]
         k_avg, w_avg = sum(kms) / steps, sum(walls) / steps
         row = {"ncount": ncount, "gated": gate, "launches": launches, "launch_shape": shape,
                "ms_kernel": k_avg, "ms_wall": w_avg,
                "neutrons_per_s_kernel": ncount / (k_avg * 1e-3), "neutrons_per_s_wall": ncount / (w_avg * 1e-3),
                "falg_ops_per_neutron": fl["F_alg"], "draws_per_neutron": fl["draws"],
                "fp64_frac": fl["F_alg"] * ncount / (k_avg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-               "cpu_all_cores": cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, gate=gate)}
+               "cpu_all_cores": m["cpu"]}
         if cpu_one_core:
-            row["cpu_one_core"] = cpu_baseline(configs.BUILDERS[chain](), ncount, seconds=cpu_seconds, cores=1,
-                                               gate=gate)
+            row["cpu_one_core"] = m["cpu1"]
         rows[name] = row
     return rows
 
@@ -295,6 +305,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2s", choices=sorted(WORKLOADS))
     ap.add_argument("--ncount", type=float, default=None)
+    ap.add_argument("--buffer-size", type=float, default=0,
+                    help="neutrons per launch (whole streams; 0: one launch), as run_script.run's buffer_size")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--falg-neutrons", type=int, default=1000000)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
@@ -341,14 +353,15 @@ def main():
                  "trials_ms": {"%d/%s/%s" % (t["block"], t["waves_per_eu"], t["streams_per_lane"]): round(t["ms"], 2)
                                for t in best["trials"]}}
     seed = 0
+    buf = int(a.buffer_size)
     for _ in range(a.warmup):
         h.clear()
-        h.run(ncount, seed, 0, 0, MCV_FLAG_GATE_DEAD)
+        h.run(ncount, seed, 0, 0, MCV_FLAG_GATE_DEAD, buffer_size=buf)
     kernel_ms, rng_ms = [], []
     h.clear()
     t0 = time.perf_counter()
     for k in range(a.steps):
-        st = h.run(ncount, seed + k, 0, 0, MCV_FLAG_GATE_DEAD)
+        st = h.run(ncount, seed + k, 0, 0, MCV_FLAG_GATE_DEAD, buffer_size=buf)
         kernel_ms.append(st["ms_kernel"])
         rng_ms.append(st["ms_rng_init"])
     elapsed = time.perf_counter() - t0
@@ -371,7 +384,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "neutrons/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{a.workload}: {desc}", "ncount_per_step": ncount,
+        "config": {"workload": f"{a.workload}: {desc}", "ncount_per_step": ncount, "buffer_size": buf or None,
                    "partition": "reference default (ntotalthreads 1e6, 512 threads/block)",
                    "semantics": "dead neutrons (p<=0) dropped", "launch_shape": shape},
         "roofline": roofline, "cpu_baseline": cpu,
