@@ -56,6 +56,7 @@ EXPORTED = (
     "mcv_instr_set_params", "mcv_instr_run", "mcv_monitor_read", "mcv_monitor_clear",
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
+    "mcv_batch_histograms_read",
     "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process", "mcv_scratch_fit",
     "mcv_instr_run_noalloc",
 )
@@ -153,6 +154,8 @@ def lib():
                                            ctypes.POINTER(McvRunStats)]
     L.mcv_batch_monitor_read.restype = ctypes.c_int
     L.mcv_batch_monitor_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, dp]
+    L.mcv_batch_histograms_read.restype = ctypes.c_int
+    L.mcv_batch_histograms_read.argtypes = [ctypes.c_void_p, dp, ctypes.c_int64]
     L.mcv_batch_clear.restype = ctypes.c_int
     L.mcv_batch_clear.argtypes = [ctypes.c_void_p]
     L.mcv_batch_destroy.restype = None
@@ -364,6 +367,12 @@ class BatchHandle:
         out = np.zeros(3 * self.nbins[comp])
         check(lib().mcv_batch_monitor_read(self.h, int(inst), int(comp), _dptr(out)))
         return out.reshape(3, -1)
+
+    def read_histograms(self):
+        """(ninst, H) array: every instance's (3, nbins) monitor blocks in chain order, one copy."""
+        out = np.empty((self.ninst, 3 * sum(self.nbins)))
+        check(lib().mcv_batch_histograms_read(self.h, _dptr(out), out.size))
+        return out
 
     def clear(self):
         check(lib().mcv_batch_clear(self.h))

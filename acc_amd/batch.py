@@ -88,16 +88,21 @@ class BatchedScan:
         return self.handle.read_monitor(inst, comp)
 
     def store(self, accumulate=False):
-        """Copy every instance's histograms into its monitors' `out` arrays."""
+        """Copy every instance's histograms into its monitors' `out` arrays
+        (one device-to-host copy of all of them)."""
         from .components.monitors.MonitorBase import MonitorBase
+        allh = self.handle.read_histograms()
+        nb = self.handle.nbins
         for k, instr in enumerate(self.instruments):
+            off = 0
             for i, c in enumerate(instr.components):
-                if isinstance(c, MonitorBase):
-                    h = self.handle.read_monitor(k, i).reshape(c.out.shape)
+                if nb[i] > 0 and isinstance(c, MonitorBase):
+                    h = allh[k, off:off + 3 * nb[i]].reshape(c.out.shape)
                     if accumulate:
                         c.out += h
                     else:
                         c.out[...] = h
+                off += 3 * nb[i]
 
     def close(self):
         self.handle.close()

@@ -2015,6 +2015,15 @@ int mcv_batch_monitor_read(mcv_batch* b, int32_t inst, int32_t comp, double* out
   return MCV_OK;
 }
 
+int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n) {
+  if (!b || (!out && n)) return fail(MCV_ERR_ARG, "bad batch or output buffer");
+  if (n != b->hstride * b->ninst)
+    return fail(MCV_ERR_ARG, "histograms of the batch hold %lld doubles, not %lld",
+                (long long)(b->hstride * b->ninst), (long long)n);
+  if (n > 0) HIPCHECK(hipMemcpy(out, b->d_hist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return MCV_OK;
+}
+
 int mcv_batch_clear(mcv_batch* b) {
   if (!b) return fail(MCV_ERR_ARG, "null batch");
   const int64_t n = b->hstride * b->ninst;

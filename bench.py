@@ -98,9 +98,16 @@ def cpu_baseline(instr, ncount, seconds=12.0, ntotalthreads=0, tpb=0, cores=None
 
 def batch_scan(ninst=64, reps=3):
     """The 64-instance scan (Ei 20-200 meV log-uniform, sample rotation
-    0-180 deg, counts log-uniform 1e5-1e7, default_rng(0)): one persistent
-    hipGraph launch vs the same instances launched back to back, and vs a
-    loop of run_instrument calls (the one-run-per-instrument API)."""
+    0-180 deg, counts log-uniform 1e5-1e7, default_rng(0)) two ways, each
+    like for like:
+     * on the device: one persistent hipGraph launch of the batch kernel vs
+       the same instances launched back to back (stream seeding + kernel per
+       instance, no host work between), both timed by HIP events and by wall
+       clock around the launches;
+     * end to end through the Python API, histograms delivered into every
+       instance's monitor `out` arrays: BatchedScan.run + store() vs a loop of
+       run_instrument calls (the one-run-per-instrument API).
+    Every run checks every instance's processed count."""
     from acc_amd import configs
     from acc_amd.batch import BatchedScan
     from acc_amd.run_script import run_instrument
@@ -109,6 +116,7 @@ def batch_scan(ninst=64, reps=3):
     total = int(n.sum())
     scan, kw = tuned_scan(ins, n, use_cache=False)
     tuned = {"block": scan.block, "waves_per_eu": scan.waves_per_eu, "streams_per_lane": scan.streams_per_lane}
+
     def checked(st):
         assert st["processed"] == total, st  # (the runtime also checks every instance's count)
         return st
@@ -126,6 +134,12 @@ def batch_scan(ninst=64, reps=3):
         for _ in range(reps):
             checked(scan.run_sequential(n, seed=0))
         s_wall = (time.perf_counter() - t) / reps * 1e3
+        scan.store()
+        t = time.perf_counter()
+        for _ in range(reps):
+            checked(scan.run(n, seed=0, **kw))
+            scan.store()
+        b_e2e = (time.perf_counter() - t) / reps * 1e3
     finally:
         scan.close()
     from acc_amd import config
@@ -137,10 +151,10 @@ def batch_scan(ninst=64, reps=3):
     config.kernel_streams_per_lane = tuned["streams_per_lane"]
     try:
         for instr, c in zip(singles, n):  # builds the handles (module loads) outside the timing
-            run_instrument(instr, int(min(c, 1000)), seed=0)
+            run_instrument(instr, int(min(c, 1000)), seed=0, accumulate=False)
         t = time.perf_counter()
         for instr, c in zip(singles, n):
-            assert run_instrument(instr, int(c), seed=0)["processed"] == c
+            assert run_instrument(instr, int(c), seed=0, accumulate=False)["processed"] == c
         api_wall = (time.perf_counter() - t) * 1e3
     finally:
         config.kernel_block, config.kernel_waves_per_eu = saved
@@ -148,10 +162,12 @@ def batch_scan(ninst=64, reps=3):
     bk = sum(b) / len(b)
     return {"instances": ninst, "neutrons": total, "ms_batched_graph_kernel": bk, "ms_batched_graph_wall": b_wall,
             "ms_back_to_back_kernels": sum(s) / len(s), "ms_back_to_back_wall": s_wall,
-            "ms_run_instrument_loop_wall": api_wall,
-            "speedup_vs_back_to_back": s_wall / b_wall, "speedup_vs_run_instrument_loop": api_wall / b_wall,
+            "speedup_vs_back_to_back": s_wall / b_wall,
+            "ms_batched_with_histograms_wall": b_e2e, "ms_run_instrument_loop_wall": api_wall,
+            "speedup_end_to_end": api_wall / b_e2e,
             "neutrons_per_s_batched": total / (b_wall * 1e-3), "launch_shape": tuned,
-            "note": "same tuned kernel in all three; back-to-back = one launch per instance, no host work between"}
+            "note": "same tuned kernel everywhere; back-to-back = one launch per instance, no host work between; "
+                    "end to end = histograms of every instance copied into its monitors' out arrays"}
 
 
 def adjoint_cost(ncount=int(1e7), reps=3):

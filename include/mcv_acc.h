@@ -284,6 +284,12 @@ int mcv_batch_run_sequential(mcv_batch* b, const uint64_t* ncounts, uint64_t see
                              int64_t ntotalthreads, int32_t tpb, uint32_t flags, mcv_run_stats* stats);
 
 int mcv_batch_monitor_read(mcv_batch* b, int32_t inst, int32_t comp, double* out3);
+
+/* Every instance's histograms in one copy: out holds n = ninst x H doubles,
+ * H = 3 x (sum of the monitors' bins), instance-major, each instance's
+ * monitors in chain order as mcv_batch_monitor_read returns them (one
+ * device-to-host copy instead of one per instance and monitor). */
+int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n);
 int mcv_batch_clear(mcv_batch* b);
 void mcv_batch_destroy(mcv_batch* b);
 

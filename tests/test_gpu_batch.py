@@ -80,6 +80,28 @@ def test_batch_graph_replays(scan_and_expected):
         _check(scan, exp)
 
 
+def test_batch_histograms_in_one_copy(scan_and_expected):
+    """mcv_batch_histograms_read returns the blocks mcv_batch_monitor_read
+    does, and store() fills every instance's monitor out arrays with them."""
+    scan, exp = scan_and_expected
+    n = [p[2] for p in PLAN]
+    scan.run(n, seed=SEED)
+    allh = scan.handle.read_histograms()
+    nb = scan.handle.nbins
+    assert allh.shape == (len(PLAN), 3 * sum(nb))
+    for k in range(len(PLAN)):
+        off = 0
+        for i, b in enumerate(nb):
+            if b > 0:
+                np.testing.assert_array_equal(allh[k, off:off + 3 * b].reshape(3, -1), scan.monitor(k, i))
+            off += 3 * b
+    scan.store()
+    for k, instr in enumerate(scan.instruments):
+        for i, c in enumerate(instr.components):
+            if nb[i] > 0:
+                np.testing.assert_array_equal(c.out.reshape(3, -1), scan.monitor(k, i))
+
+
 def test_batch_sequential_matches(scan_and_expected):
     scan, exp = scan_and_expected
     n = [p[2] for p in PLAN]
