@@ -83,7 +83,7 @@ __device__ __forceinline__ double div_recip(double a, double b, double y) {
 // Monitors own (3, nbins) float64 blocks: N, sum p, sum p^2, updated with
 // device-scope FP64 atomic adds that return nothing (the wave does not wait
 // for them).  With eight XCDs each keeping its own L2, a device-scope atomic
-// is performed past L2: rocprofv3 counts one 64-byte memory write per add.
+// is performed past L2: rocprofv3 counts one 32-byte memory write per add.
 __device__ __forceinline__ void hist_add3(double* __restrict__ H, int64_t nbins, int64_t b, double p) {
   unsafeAtomicAdd(H + b, 1.0);
   unsafeAtomicAdd(H + nbins + b, p);
