@@ -249,20 +249,27 @@ __device__ __forceinline__ void count_processed(unsigned long long* counter, int
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(counter, v);
 }
 
-// Neutrons of stream tid: [tid*npt, min((tid+1)*npt, N)), or empty for a
-// stream outside this launch's range (a run split into launches by stream
-// ranges computes every stream exactly as one launch over all of them does).
-__device__ __forceinline__ void stream_range(const LaunchArgs& A, int64_t tid, Rng& rng, int64_t& idx,
+// Neutrons of stream tid: [tid*npt, min((tid+1)*npt, N)), or this launch's
+// slice of them, or empty for a stream outside this launch's range (a run
+// split into launches by stream ranges, or by slices of every stream with the
+// states carried between launches, computes every stream exactly as one
+// launch over all of them does).  Returns whether the stream is in range.
+__device__ __forceinline__ bool stream_range(const LaunchArgs& A, int64_t tid, Rng& rng, int64_t& idx,
                                              int64_t& end) {
   idx = end = 0;
   if (tid < A.nthreads && tid < A.tid_end) {
     const int64_t r = tid - A.rng_first;
     rng.s0 = A.rng[2 * r];
     rng.s1 = A.rng[2 * r + 1];
-    idx = tid * A.npt;
-    end = idx + A.npt < A.N ? idx + A.npt : A.N;
+    const int64_t first = tid * A.npt;
+    int64_t e = first + A.npt;
+    idx = first + A.slice0;
+    if (A.slice_len > 0 && idx + A.slice_len < e) e = idx + A.slice_len;
+    end = e < A.N ? e : A.N;
     if (end < idx) end = idx;
+    return true;
   }
+  return false;
 }
 
 // ------------------------------------------------------------------ plain chains
@@ -302,12 +309,17 @@ __device__ __forceinline__ void run_lanes(const LaunchArgs& A) {
        first += (int64_t)gridDim.x * MCV_BLOCK) {
     Rng rng;
     int64_t idx, end;
-    stream_range(A, first + threadIdx.x, rng, idx, end);
+    const bool live = stream_range(A, first + threadIdx.x, rng, idx, end);
     for (int64_t i = idx; i < end; ++i) {
       const int alive = (int)Chain::body(rng, n, A, i);
       dump_neutron(A, i, n, alive);
     }
     mine += end - idx;
+    if (A.rng_out != nullptr && live) {  // the next slice continues this stream
+      const int64_t r = first + threadIdx.x - A.rng_first;
+      A.rng_out[2 * r] = rng.s0;
+      A.rng_out[2 * r + 1] = rng.s1;
+    }
   }
   count_processed(A.counter, mine);
   lds_hist_flush(A);
@@ -712,6 +724,7 @@ __device__ __forceinline__ bool next_item(const BatchArgs& B, LaunchArgs& A, int
 
 #define MCV_DEFINE_KERNELS_PLAIN(NAME, CHAIN)                                                  \
   extern "C" __device__ int mcv_lds_cache = MCV_LDS_CACHE;                                    \
+  extern "C" __device__ int mcv_slices = 1;                                                   \
   extern "C" __global__ void MCV_KERNEL_ATTR NAME(const mcv::LaunchArgs A) {                  \
     mcv::run_lanes<CHAIN>(A);                                                                 \
   }                                                                                           \

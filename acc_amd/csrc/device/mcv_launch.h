@@ -29,6 +29,13 @@ struct LaunchArgs {
   int64_t tid_base;                  // first stream of this launch
   int64_t tid_end;                   // streams of this launch: [tid_base, min(tid_end, nthreads))
   int64_t rng_first;                 // stream whose state is rng[0..1]
+  // Slice of every stream this launch runs: neutrons [tid*npt + slice0,
+  // tid*npt + slice0 + slice_len) of stream tid (clipped to the stream and to
+  // N; slice_len = 0: to the stream's end).  Only plain kernels (mcv_slices =
+  // 1) take a slice_len, and then write each stream's state back to
+  // rng_out[2 * (tid - rng_first)] for the next slice.
+  int64_t slice0, slice_len;
+  uint64_t* rng_out;
   int32_t gate;                      // MCV_FLAG_GATE_DEAD
   int32_t ncomp;
   int64_t off[MCV_MAX_COMP];         // parameters of component i

@@ -865,6 +865,7 @@ struct mcv_instr {
   int block = 256;  // workgroup size of the code object's kernels
   int spl = 1;      // random streams per lane (a workgroup serves block * spl streams)
   int pool = 0;     // the kernel takes streams from a pool (persistent grid)
+  int slices = 0;   // the kernel runs slices of every stream, carrying the states (plain kernels)
   int64_t lane_scratch = 0;  // private segment bytes per lane of the fused kernel
   int per_cu = 0;       // its workgroups resident per CU
   DeviceCtx* ctx = nullptr;
@@ -1072,6 +1073,8 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
   if (rc) return rc;
   rc = module_int(h->mod, "mcv_stream_pool", 0, &h->pool);
   if (rc) return rc;
+  rc = module_int(h->mod, "mcv_slices", 0, &h->slices);
+  if (rc) return rc;
   int cache = 0;  // plain kernels run the LDS bin cache (and a grid of resident workgroups)
   rc = module_int(h->mod, "mcv_lds_cache", 0, &cache);
   if (rc) return rc;
@@ -1158,12 +1161,29 @@ int64_t streams_per_launch(const mcv_instr* h, int64_t s_used, int64_t npt, uint
   return S;
 }
 
-// One run of ncount neutrons in launches over stream ranges.  `states(S, &p)`
-// provides a device buffer for the states of S streams; `counter` holds the
-// processed count and the stream-pool counter (zeroed by the first seeding).
+// Neutrons of every stream per launch when a run of a plain kernel is split
+// by buffer_size: slices of about buffer_size / streams neutrons of every
+// stream, the states carried in place from launch to launch, so that each
+// launch is as wide as the whole run (ranges of whole streams would leave a
+// launch of 1e6 neutrons at 10 per stream with 1e5 lanes).  0: split by
+// stream ranges instead (other kernels, one slice would be the whole stream,
+// or the states of all streams do not fit `max_states` or one launch).
+int64_t slice_per_launch(const mcv_instr* h, int64_t s_used, int64_t npt, uint64_t buffer_size, uint64_t ncount,
+                         int64_t max_states) {
+  if (!h->slices || buffer_size == 0 || buffer_size >= ncount || npt < 2 || s_used > max_states) return 0;
+  if (streams_per_launch(h, s_used, npt, 0) != s_used) return 0;
+  const int64_t k = std::max<int64_t>(1, (int64_t)(buffer_size / (uint64_t)s_used));
+  return k < npt ? k : 0;
+}
+
+// One run of ncount neutrons in launches over stream ranges or stream slices.
+// `states(S, &p)` provides a device buffer for the states of S streams (at
+// most `max_states`); `counter` holds the processed count and the stream-pool
+// counter (zeroed by the first seeding).
 template <class States>
 int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
-             uint64_t buffer_size, mcv_run_stats* stats, States states, unsigned long long* counter) {
+             uint64_t buffer_size, mcv_run_stats* stats, States states, unsigned long long* counter,
+             int64_t max_states = INT64_MAX) {
   const auto t0 = std::chrono::steady_clock::now();
   int64_t nblocks, nthreads, npt;
   partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
@@ -1177,12 +1197,13 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
   if (ncount == 0) return MCV_OK;
   // streams past the last one holding neutrons run nothing
   const int64_t s_used = std::min<int64_t>(nthreads, (int64_t)((ncount + (uint64_t)npt - 1) / (uint64_t)npt));
-  const int64_t S = streams_per_launch(h, s_used, npt, buffer_size);
+  const int64_t slice = slice_per_launch(h, s_used, npt, buffer_size, ncount, max_states);
+  const int64_t S = slice > 0 ? s_used : streams_per_launch(h, s_used, npt, buffer_size);
   if (S <= 0) return scratch_error(h->lane_scratch, h->block, h->ctx);
   uint64_t* d_states = nullptr;
   int rc = states(S, &d_states);
   if (rc) return rc;
-  const int64_t nchunks = (s_used + S - 1) / S;
+  const int64_t nchunks = slice > 0 ? (npt + slice - 1) / slice : (s_used + S - 1) / S;
   const int64_t per_group = (int64_t)h->block * h->spl;
   // a persistent grid: as many workgroups as are resident at once within the scratch budget
   const int64_t pool_grid =
@@ -1205,11 +1226,16 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
   for (int64_t c = 0; c < nchunks; ++c) {
     if (c >= kEvRing && (rc = harvest(c - kEvRing))) return rc;
     hipEvent_t* e = h->rev + 3 * (c % kEvRing);
-    const int64_t s0 = c * S, s1 = std::min<int64_t>(s0 + S, s_used);
+    const int64_t s0 = slice > 0 ? 0 : c * S, s1 = std::min<int64_t>(s0 + S, s_used);
     HIPCHECK(hipEventRecord(e[0], h->stream));
-    // the first launch also zeroes the processed count; every launch its stream pool
-    rc = seed_streams(h->ctx, h->stream, seed, s0, s1 - s0, d_states, c == 0 ? counter : counter + 1, c == 0 ? 2 : 1);
-    if (rc) return rc;
+    // the first launch also zeroes the processed count; every launch of
+    // stream ranges seeds its streams and zeroes its stream pool; slices
+    // continue the states the previous launch left
+    if (slice == 0 || c == 0) {
+      rc = seed_streams(h->ctx, h->stream, seed, s0, s1 - s0, d_states, c == 0 ? counter : counter + 1,
+                        c == 0 ? 2 : 1);
+      if (rc) return rc;
+    }
     HIPCHECK(hipEventRecord(e[1], h->stream));
     mcv::LaunchArgs A = h->args;
     A.P = h->d_blob;
@@ -1228,6 +1254,9 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
     A.npt = npt;
     A.tid_base = s0;
     A.tid_end = s1;
+    A.slice0 = slice > 0 ? c * slice : 0;
+    A.slice_len = slice;
+    A.rng_out = slice > 0 ? d_states : nullptr;
     A.gate = (flags & MCV_FLAG_GATE_DEAD) ? 1 : 0;
     int64_t grid = (s1 - s0 + per_group - 1) / per_group;
     if (h->pool) grid = std::min(grid, pool_grid);
@@ -1347,7 +1376,7 @@ int mcv_instr_run_noalloc(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t 
     return MCV_OK;
   };
   return run_impl(h, ncount, seed, ntotalthreads, tpb, flags, buffer_size, stats, states,
-                  reinterpret_cast<unsigned long long*>(base));
+                  reinterpret_cast<unsigned long long*>(base), kScratchStreams);
 }
 
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3) {

@@ -1,8 +1,10 @@
 """Runs split into launches: `buffer_size` (run_script.run's keyword) and the
 scratch guard split a run into launches over ranges of whole streams of the
-one reference partition, each stream seeded as in a single launch, so every
-neutron's final state and every histogram count must equal the single
-launch's (weight sums up to the order of atomic additions).
+one reference partition, each stream seeded as in a single launch (plain
+kernels: over slices of every stream, each launch continuing the states the
+previous one left), so every neutron's final state and every histogram count
+must equal the single launch's (weight sums up to the order of atomic
+additions).
 
 Also: seeding a stream range directly, the scratch guard's narrowing and
 refusal, and a handle used for buffered processing, the adjoint and
@@ -40,10 +42,11 @@ def _same(a, b):
         np.testing.assert_allclose(h1[i][1:], h0[i][1:], rtol=1e-12, atol=1e-300)
 
 
-# plain (c1), compacted (c2), stream-carrying pooled (c2s), multiple scattering (c3-small)
-@pytest.mark.parametrize("name,N,ntot,buf", [("c1", 400000, 20000, 57000), ("c2", 300000, 20000, 61000),
-                                             ("c2s", 300000, 20000, 45000), ("c3-small", 60000, 4096, 9000),
-                                             ("c2", 300000, 20000, 7)])
+# plain (c1: slices of every stream; the last case's slice is one neutron),
+# compacted (c2), stream-carrying pooled (c2s), multiple scattering (c3-small)
+@pytest.mark.parametrize("name,N,ntot,buf", [("c1", 400000, 20000, 57000), ("c1", 399999, 20000, 1000),
+                                             ("c2", 300000, 20000, 61000), ("c2s", 300000, 20000, 45000),
+                                             ("c3-small", 60000, 4096, 9000), ("c2", 300000, 20000, 7)])
 def test_chunked_run_equals_one_launch(name, N, ntot, buf):
     print(f"\n[chunks] {name} N={N} ntot={ntot} buffer_size={buf}", flush=True)
     instr = prebuilt.instruments()[name]()
@@ -52,9 +55,13 @@ def test_chunked_run_equals_one_launch(name, N, ntot, buf):
     many = _run(h, instr, N, ntot, 4, buf)
     npt = one[0]["npt"]
     streams = -(-N // npt)
-    per = max(1, buf // npt)
     assert one[0]["nchunks"] == 1
-    assert many[0]["nchunks"] == -(-streams // per), many[0]
+    if name == "c1":  # slices of max(1, buf // streams) neutrons of every stream
+        per = max(1, buf // streams)
+        assert many[0]["nchunks"] == -(-npt // per), many[0]
+    else:  # ranges of max(1, buf // npt) whole streams
+        per = max(1, buf // npt)
+        assert many[0]["nchunks"] == -(-streams // per), many[0]
     _same(one, many)
 
 
