@@ -96,7 +96,7 @@ def run_instrument(instrument, ncount, ntotalthreads=None, threads_per_block=Non
     """Run ncount neutrons through an in-memory instrument; monitor `out`
     arrays receive the device histograms (added, as the reference's numba
     round trip of `out` accumulates).  buffer_size: neutrons per launch
-    (whole streams; None or 0: one launch)."""
+    (None or 0: one launch; the results are those of one launch)."""
     from .components.monitors.MonitorBase import MonitorBase
     h = get_handle(instrument)
     seed = config.rng_seed if seed is None else seed

@@ -172,11 +172,14 @@ int mcv_instr_set_params(mcv_instr* h, int32_t comp, const double* params,
  * until mcv_monitor_clear.  ntotalthreads/tpb <= 0 select 1e6 / 512
  * (acc/config.py:1-2).  flags: MCV_FLAG_*.
  * buffer_size (run_script.run's keyword, which the reference accepts and
- * ignores) > 0 splits the run into launches of whole streams holding about
- * buffer_size neutrons (at least one stream): launch c runs streams
- * [c*S, (c+1)*S) of the same partition, seeded as in one launch, so every
- * neutron is computed exactly as in a single launch.  The runtime also splits
- * a launch whose lanes' scratch would exceed the budget. */
+ * ignores) > 0 splits the run into launches of about buffer_size neutrons:
+ * for a chain compiled as a plain kernel, launch c runs neutrons [c*k,
+ * (c+1)*k) of every stream (k = buffer_size / streams, at least 1) and keeps
+ * each stream's state for the next launch; otherwise launch c runs whole
+ * streams [c*S, (c+1)*S) of the same partition (S = buffer_size / npt, at
+ * least 1), seeded as in one launch.  Either way every neutron is computed
+ * exactly as in a single launch.  The runtime also splits a launch whose
+ * lanes' scratch would exceed the budget. */
 int mcv_instr_run(mcv_instr* h, uint64_t ncount, uint64_t seed,
                   int64_t ntotalthreads, int32_t tpb, uint32_t flags,
                   uint64_t buffer_size, mcv_run_stats* stats);
