@@ -24,7 +24,7 @@ class DeviceFunction:
     """
 
     def __init__(self, call, stochastic=False, monitor=False, preamble=None, adjoint=None, dparams=None,
-                 binning=None, split=None):
+                 binning=None, split=None, opening=None):
         self.call = call
         self.stochastic = stochastic
         self.monitor = monitor
@@ -42,6 +42,12 @@ class DeviceFunction:
         # form, as `adjoint` is of `call`); a compacted kernel may end its
         # per-stream phase after `enter`
         self.split = split
+        # a component that turns many neutrons away at its opening without
+        # changing them otherwise: (`admits`, a bool expression that is false
+        # exactly when the component would only apply `reject`, the
+        # statement(s) it then applies, e.g. p = 0); without dead-neutron
+        # gating a compacted kernel queues only the neutrons it admits
+        self.opening = opening
 
     def render(self, comp, i, n="n"):
         return self.call.format(i=i, n=n, **comp._render_context(i))
@@ -51,6 +57,9 @@ class DeviceFunction:
 
     def render_split(self, comp, i, part, n="n", s=None):
         return self.split[part].format(i=i, n=n, s=s, **comp._render_context(i))
+
+    def render_opening(self, comp, i, part, n="n"):
+        return self.opening[part].format(i=i, n=n, **comp._render_context(i))
 
     def render_binning(self, comp, i, n="n"):
         return self.binning.format(i=i, n=n, **comp._render_context(i))

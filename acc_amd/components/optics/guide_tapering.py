@@ -85,7 +85,8 @@ class Guide(ComponentBase):
         "mcv::guide_tapering({n}, P + A.off[{i}], P + A.doff[{i}]);",
         adjoint="mcv::guide_tapering_t({n}, P + A.off[{i}], P + A.doff[{i}], mcv::TaperScore({s}));",
         dparams=lambda comp: [("R0", 1), ("Qcx", 2), ("Qcy", 3), ("alphax", 4), ("alphay", 5),
-                              ("mx", 6), ("my", 7), ("W", 8)])
+                              ("mx", 6), ("my", 7), ("W", 8)],
+        opening=("mcv::taper_admits({n}, P + A.off[{i}], P + A.doff[{i}])", "{n}.p = 0;"))
 
     def _mcv_desc(self):
         nseg, scalars, data = self.propagate_params

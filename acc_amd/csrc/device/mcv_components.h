@@ -349,33 +349,43 @@ __device__ __forceinline__ bool taper_reflect(TaperState& s, const double* __res
   return true;
 }
 
+// A segment entrance (guide_tapering.py:170-186): moves (x, y, z) to the
+// segment's entrance plane (t is left alone, as the reference leaves it) and
+// returns whether the neutron dies there: outside the opening (p = 0), or
+// beyond the channel after the shift.  x1 and edge are the shift's values.
+// Both tests are evaluated without short-circuit branches.  After the shift x
+// is in [0, w1s] (rounding keeps x > -w1s/2 from going negative), so floor(x /
+// w1s) is 0 unless the quotient rounds up to 1: divide only then.
+__device__ __forceinline__ bool taper_entrance(const double* __restrict__ rec, double& x, double& y, double& z,
+                                               double vx, double vy, double vz, double& x1, double& edge) {
+  const double w1s = rec[0], hhs = rec[1];
+  const double dt = (rec[6] - z) / vz;
+  x += vx * dt;
+  y += vy * dt;
+  z += vz * dt;
+  const bool outside = (x <= -0.5 * w1s) | (x >= 0.5 * w1s) | (y <= -hhs) | (y >= hhs);
+  x1 = x + 0.5 * w1s;
+  edge = x1 < 0.75 * w1s ? 0.0 : floor(x1 / w1s) * w1s;
+  return outside | (x1 - edge > w1s);
+}
+
 // One pass; returns true when the neutron has left the guide or died.
 template <class T>
 __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restrict__ P,
                                            const double* __restrict__ D, T& tr) {
   const double l_seg = P[0];
   const int nseg = (int)P[9];
-  if (s.enter) {  // segment entrance: guide_tapering.py:170-186
+  if (s.enter) {  // segment entrance
     const double* __restrict__ rec = D + 16 * s.seg;
-    const double w1s = rec[0], hhs = rec[1];
+    const double w1s = rec[0];
     s.wws = rec[2];
     s.hhh = rec[3];
     s.lwh = rec[4];
     s.lhh = rec[5];
     s.zr = rec[6];
     s.zend = rec[7];
-    const double dt = (s.zr - s.z) / s.vz;
-    s.x += s.vx * dt;
-    s.y += s.vy * dt;
-    s.z += s.vz * dt;
-    // the opening (p = 0 outside), then the channel shift; both tests are
-    // evaluated without short-circuit branches.  After the shift x is in
-    // [0, w1s] (rounding keeps x > -w1s/2 from going negative), so floor(x /
-    // w1s) is 0 unless the quotient rounds up to 1: divide only then
-    const bool outside = (s.x <= -0.5 * w1s) | (s.x >= 0.5 * w1s) | (s.y <= -hhs) | (s.y >= hhs);
-    const double x1 = s.x + 0.5 * w1s;
-    const double edge = x1 < 0.75 * w1s ? 0.0 : floor(x1 / w1s) * w1s;
-    if (outside | (x1 - edge > w1s)) {
+    double x1, edge;
+    if (taper_entrance(rec, s.x, s.y, s.z, s.vx, s.vy, s.vz, x1, edge)) {
       s.died = true;
       return true;
     }
@@ -442,6 +452,19 @@ __device__ __forceinline__ bool taper_step(TaperState& s, const double* __restri
     if (!s.enter) return true;
   }
   return false;
+}
+
+// Whether the guide lets the neutron through its first segment's opening:
+// false exactly when guide_tapering would leave it as it came with p = 0 at
+// that opening (the same arithmetic on a copy).  Unchanged neutrons make this
+// a test a kernel can run before deciding where the guide runs.
+__device__ __forceinline__ bool taper_admits(const Neutron& n, const double* __restrict__ P,
+                                             const double* __restrict__ D) {
+  if ((int)P[9] <= 0) return true;
+  TaperState s;
+  taper_init(s, n, P);
+  double x1, edge;
+  return !taper_entrance(D, s.x, s.y, s.z, s.vx, s.vy, s.vz, x1, edge);
 }
 
 // A dead neutron keeps the state it entered with (p = 0); a transmitted one

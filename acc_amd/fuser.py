@@ -158,6 +158,31 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
     return src.replace('#include "mcv_kernel.h"', defs + '#include "mcv_kernel.h"', 1), structure
 
 
+def _ungated_opening(comps, k):
+    """Statements ending phase A of an ungated run for a neutron that
+    component k + 1 turns away at its opening: that component's reject (the
+    only change it would make), then the rest of the chain, in phase A; an
+    admitted neutron is queued unchanged (phase B applies the frame change
+    and runs component k + 1 in full).  Without gating every neutron runs
+    through every component, so phase B would otherwise hold every neutron,
+    most of which the next component's opening stops at once."""
+    if k + 1 >= len(comps) or comps[k + 1].propagate.opening is None:
+        return ""
+    nxt = comps[k + 1]
+    tail = _phase_lines(comps, k + 2, len(comps), True)
+    return f"""
+    {{  // the next component's opening
+      mcv::Neutron m = n;
+      mcv::apply_transformation(m, P + A.xoff[{k}]);
+      if (!{nxt.propagate.render_opening(nxt, k + 1, 0, n="m")}) {{
+        n = m;
+        {nxt.propagate.render_opening(nxt, k + 1, 1)}
+        {chr(10).join("        " + line for line in tail).strip()}
+        return -1;
+      }}
+    }}"""
+
+
 def _split_phases(comps, k):
     """Phases A and B when component k can stop after its entrance: with
     gating (dead neutrons dropped), phase A ends with the entrance and phase B
@@ -165,7 +190,8 @@ def _split_phases(comps, k):
     wavefronts of neutrons that entered; without gating (every neutron runs
     through every component, absorbed or not) phase A runs all of component k
     as before, since a neutron absorbed at the entrance must then not run the
-    rest."""
+    rest, and then, when component k + 1 has an opening test, queues only the
+    neutrons that test admits."""
     c = comps[k]
     a = _phase_lines(comps, 0, k, False)
     if k > 0:
@@ -173,7 +199,7 @@ def _split_phases(comps, k):
     a.append(f"""if (A.gate) {{
     if (!{c.propagate.render_split(c, k, 0)}) return 0;
   }} else {{
-    {c.propagate.render(c, k)}
+    {c.propagate.render(c, k)}{_ungated_opening(comps, k)}
   }}""")
     a.append("if (A.gate && !(n.p > 0)) return 0;")
     a.append("return 1;")
