@@ -57,6 +57,7 @@ EXPORTED = (
     "mcv_instr_set_dump", "mcv_instr_read_dump", "mcv_rng_states", "mcv_instr_destroy",
     "mcv_batch_create", "mcv_batch_run", "mcv_batch_run_sequential", "mcv_batch_monitor_read",
     "mcv_batch_histograms_read",
+    "mcv_batch_histograms_deliver",
     "mcv_batch_clear", "mcv_batch_destroy", "mcv_instr_adjoint", "mcv_instr_process", "mcv_scratch_fit",
     "mcv_instr_run_noalloc",
 )
@@ -156,6 +157,10 @@ def lib():
     L.mcv_batch_monitor_read.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, dp]
     L.mcv_batch_histograms_read.restype = ctypes.c_int
     L.mcv_batch_histograms_read.argtypes = [ctypes.c_void_p, dp, ctypes.c_int64]
+    L.mcv_batch_histograms_deliver.restype = ctypes.c_int
+    L.mcv_batch_histograms_deliver.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), ctypes.c_int32]
     L.mcv_batch_clear.restype = ctypes.c_int
     L.mcv_batch_clear.argtypes = [ctypes.c_void_p]
     L.mcv_batch_destroy.restype = None
@@ -253,8 +258,15 @@ class Handle:
         p = np.ascontiguousarray(params, dtype=np.float64)
         check(lib().mcv_instr_set_params(self.h, int(comp), _dptr(p), p.size))
 
-    def read_monitor(self, comp):
-        out = np.zeros(3 * self.nbins[comp])
+    def read_monitor(self, comp, out=None):
+        """Monitor comp's (3, nbins) histogram: a new array, or written into
+        `out` (a writeable C-contiguous float64 array of 3 x nbins values)."""
+        if out is None:
+            out = np.empty(3 * self.nbins[comp])
+        elif (out.dtype != np.float64 or not out.flags.c_contiguous or not out.flags.writeable
+              or out.size != 3 * self.nbins[comp]):
+            raise McvError(MCV_ERR_ARG, "monitor %d: out must be a writeable C-contiguous float64 array of %d "
+                           "values" % (comp, 3 * self.nbins[comp]))
         check(lib().mcv_monitor_read(self.h, int(comp), _dptr(out)))
         return out.reshape(3, -1)
 
@@ -373,6 +385,21 @@ class BatchHandle:
         out = np.empty((self.ninst, 3 * sum(self.nbins)))
         check(lib().mcv_batch_histograms_read(self.h, _dptr(out), out.size))
         return out
+
+    def deliver_histograms(self, blocks, accumulate=False):
+        """Writes (or adds) histogram blocks into arrays: blocks is a list of
+        (offset, dst) with offset in read_histograms' flattened layout and dst a
+        C-contiguous float64 array receiving dst.size doubles."""
+        nb = len(blocks)
+        src = (ctypes.c_int64 * nb)(*[int(o) for o, _ in blocks])
+        ln = (ctypes.c_int64 * nb)()
+        dst = (ctypes.POINTER(ctypes.c_double) * nb)()
+        for j, (_, d) in enumerate(blocks):
+            if d.dtype != np.float64 or not d.flags.c_contiguous or not d.flags.writeable:
+                raise McvError(MCV_ERR_ARG, "block %d: destination must be a writeable C-contiguous float64 array" % j)
+            ln[j] = d.size
+            dst[j] = _dptr(d)
+        check(lib().mcv_batch_histograms_deliver(self.h, nb, src, ln, dst, 1 if accumulate else 0))
 
     def clear(self):
         check(lib().mcv_batch_clear(self.h))

@@ -88,21 +88,33 @@ class BatchedScan:
         return self.handle.read_monitor(inst, comp)
 
     def store(self, accumulate=False):
-        """Copy every instance's histograms into its monitors' `out` arrays
-        (one device-to-host copy of all of them)."""
+        """Copy (or add, accumulate=True) every instance's histograms into its
+        monitors' `out` arrays: one device-to-host copy of all of them, then
+        each block written straight into its array by the runtime."""
         from .components.monitors.MonitorBase import MonitorBase
-        allh = self.handle.read_histograms()
         nb = self.handle.nbins
+        H = 3 * sum(nb)
+        blocks, staged = [], []
         for k, instr in enumerate(self.instruments):
-            off = 0
+            off = k * H
             for i, c in enumerate(instr.components):
                 if nb[i] > 0 and isinstance(c, MonitorBase):
-                    h = allh[k, off:off + 3 * nb[i]].reshape(c.out.shape)
-                    if accumulate:
-                        c.out += h
-                    else:
-                        c.out[...] = h
+                    if c.out.size != 3 * nb[i]:
+                        raise ValueError("%s: out holds %d values, the device histogram %d" %
+                                         (c.name, c.out.size, 3 * nb[i]))
+                    if c.out.dtype == np.float64 and c.out.flags.c_contiguous and c.out.flags.writeable:
+                        blocks.append((off, c.out))
+                    else:  # an out array the runtime cannot write: through a temporary
+                        tmp = np.zeros(3 * nb[i])  # added to, or written: the block's values either way
+                        blocks.append((off, tmp))
+                        staged.append((c, tmp))
                 off += 3 * nb[i]
+        self.handle.deliver_histograms(blocks, accumulate=accumulate)
+        for c, tmp in staged:
+            if accumulate:
+                c.out += tmp.reshape(c.out.shape)
+            else:
+                c.out[...] = tmp.reshape(c.out.shape)
 
     def close(self):
         self.handle.close()

@@ -17,6 +17,7 @@
 #include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mcv_acc.h"
@@ -1715,6 +1716,7 @@ struct mcv_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<int64_t> graph_key;
+  double* h_stage = nullptr;  // pinned host copy of d_hist (mcv_batch_histograms_deliver)
 };
 
 namespace {
@@ -1731,6 +1733,7 @@ void release_batch(mcv_batch* b) {
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->mod) (void)hipModuleUnload(b->mod);
+  if (b->h_stage) (void)hipHostFree(b->h_stage);
   delete b;
 }
 
@@ -2050,6 +2053,41 @@ int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n) {
     return fail(MCV_ERR_ARG, "histograms of the batch hold %lld doubles, not %lld",
                 (long long)(b->hstride * b->ninst), (long long)n);
   if (n > 0) HIPCHECK(hipMemcpy(out, b->d_hist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return MCV_OK;
+}
+
+int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* src, const int64_t* len,
+                                 double* const* dst, int32_t accumulate) {
+  if (!b || nblocks < 0 || (nblocks > 0 && (!src || !len || !dst)))
+    return fail(MCV_ERR_ARG, "bad batch or block lists");
+  const int64_t n = b->hstride * b->ninst;
+  int64_t bytes = 0;
+  for (int64_t j = 0; j < nblocks; ++j) {
+    if (src[j] < 0 || len[j] < 0 || src[j] > n - len[j] || (len[j] > 0 && !dst[j]))
+      return fail(MCV_ERR_ARG, "block %lld (offset %lld, %lld doubles) is outside the %lld histogram doubles",
+                  (long long)j, (long long)src[j], (long long)len[j], (long long)n);
+    bytes += len[j] * (int64_t)sizeof(double);
+  }
+  if (n == 0 || bytes == 0) return MCV_OK;
+  if (!b->h_stage) HIPCHECK(hipHostMalloc((void**)&b->h_stage, (size_t)n * sizeof(double), hipHostMallocDefault));
+  HIPCHECK(hipMemcpy(b->h_stage, b->d_hist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  // blocks dealt to threads in turn; a few MB per thread at least
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({8, nblocks, bytes >> 22}));
+  auto work = [&](int64_t t) {
+    for (int64_t j = t; j < nblocks; j += nt) {
+      const double* s = b->h_stage + src[j];
+      double* d = dst[j];
+      if (accumulate) {
+        for (int64_t i = 0; i < len[j]; ++i) d[i] += s[i];
+      } else {
+        std::memcpy(d, s, (size_t)len[j] * sizeof(double));
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
   return MCV_OK;
 }
 

@@ -107,6 +107,11 @@ def run_instrument(instrument, ncount, ntotalthreads=None, threads_per_block=Non
                   buffer_size=int(buffer_size or 0))
     for i, c in enumerate(instrument.components):
         if isinstance(c, MonitorBase):
+            direct = (not accumulate and c.out.dtype == np.float64 and c.out.flags.c_contiguous
+                      and c.out.flags.writeable and c.out.size == 3 * h.nbins[i])
+            if direct:  # straight into the monitor's array
+                h.read_monitor(i, out=c.out)
+                continue
             out = h.read_monitor(i).reshape(c.out.shape)
             if accumulate:
                 c.out += out

@@ -293,6 +293,17 @@ int mcv_batch_monitor_read(mcv_batch* b, int32_t inst, int32_t comp, double* out
  * monitors in chain order as mcv_batch_monitor_read returns them (one
  * device-to-host copy instead of one per instance and monitor). */
 int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n);
+
+/* Every instance's histograms delivered to the caller's arrays: block j is
+ * the len[j] doubles from offset src[j] of the layout mcv_batch_histograms_read
+ * returns, written to dst[j] (accumulate 0) or added to it (accumulate 1).
+ * One device-to-host copy into the batch's pinned staging buffer (allocated on
+ * first use, freed with the batch), then the blocks are written by up to 8
+ * host threads: what a scan's caller needs to fill each instance's monitor
+ * arrays (acc/components/monitors/*.py `out`), without a temporary copy.
+ * Blocks must not overlap in dst. */
+int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* src, const int64_t* len,
+                                 double* const* dst, int32_t accumulate);
 int mcv_batch_clear(mcv_batch* b);
 void mcv_batch_destroy(mcv_batch* b);
 

@@ -100,6 +100,17 @@ def test_batch_histograms_in_one_copy(scan_and_expected):
         for i, c in enumerate(instr.components):
             if nb[i] > 0:
                 np.testing.assert_array_equal(c.out.reshape(3, -1), scan.monitor(k, i))
+    # adding: out + h + h is 2h exactly; an out array the runtime cannot write
+    # (Fortran order here) goes through a temporary
+    c0 = [c for c in scan.instruments[0].components if nb[scan.instruments[0].components.index(c)] > 0][0]
+    c0.out = np.asfortranarray(np.zeros(c0.out.shape))
+    scan.store(accumulate=True)
+    scan.store(accumulate=True)
+    for k, instr in enumerate(scan.instruments):
+        for i, c in enumerate(instr.components):
+            if nb[i] > 0:
+                want = scan.monitor(k, i) * (2.0 if c is c0 else 3.0)
+                np.testing.assert_array_equal(np.ascontiguousarray(c.out).reshape(3, -1), want)
 
 
 def test_batch_sequential_matches(scan_and_expected):
