@@ -2085,8 +2085,15 @@ int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* s
     }
   };
   std::vector<std::thread> pool;
-  for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work, t);
-  work(0);
+  std::vector<int64_t> mine{0};  // thread indices this thread works through
+  for (int64_t t = 1; t < nt; ++t) {
+    try {
+      pool.emplace_back(work, t);
+    } catch (...) {  // no thread to be had: this one does that share too
+      mine.push_back(t);
+    }
+  }
+  for (const int64_t t : mine) work(t);
   for (auto& th : pool) th.join();
   return MCV_OK;
 }
