@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1717,6 +1718,7 @@ struct mcv_batch {
   hipGraphExec_t gexec = nullptr;
   std::vector<int64_t> graph_key;
   double* h_stage = nullptr;  // pinned host copy of d_hist (mcv_batch_histograms_deliver)
+  std::vector<hipEvent_t> stage_ev;  // one per staged piece of d_hist
 };
 
 namespace {
@@ -1734,6 +1736,7 @@ void release_batch(mcv_batch* b) {
     if (e) (void)hipEventDestroy(e);
   if (b->mod) (void)hipModuleUnload(b->mod);
   if (b->h_stage) (void)hipHostFree(b->h_stage);
+  for (auto& e : b->stage_ev) (void)hipEventDestroy(e);
   delete b;
 }
 
@@ -2070,11 +2073,34 @@ int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* s
   }
   if (n == 0 || bytes == 0) return MCV_OK;
   if (!b->h_stage) HIPCHECK(hipHostMalloc((void**)&b->h_stage, (size_t)n * sizeof(double), hipHostMallocDefault));
-  HIPCHECK(hipMemcpy(b->h_stage, b->d_hist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  // the copy goes in pieces, each followed by an event, so that blocks are
+  // written out while the later pieces are still in flight
+  constexpr int64_t kPiece = int64_t(1) << 20;  // doubles (8 MB)
+  const int64_t npiece = (n + kPiece - 1) / kPiece;
+  while ((int64_t)b->stage_ev.size() < npiece) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->stage_ev.push_back(e);
+  }
+  for (int64_t c = 0; c < npiece; ++c) {
+    const int64_t c0 = c * kPiece, cn = std::min(kPiece, n - c0);
+    HIPCHECK(hipMemcpyAsync(b->h_stage + c0, b->d_hist + c0, (size_t)cn * sizeof(double), hipMemcpyDeviceToHost,
+                            b->stream));
+    HIPCHECK(hipEventRecord(b->stage_ev[c], b->stream));
+  }
   // blocks dealt to threads in turn; a few MB per thread at least
   const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({8, nblocks, bytes >> 22}));
+  std::atomic<int> err{(int)hipSuccess};
   auto work = [&](int64_t t) {
     for (int64_t j = t; j < nblocks; j += nt) {
+      if (len[j] == 0) continue;
+      // the pieces arrive in order: the one holding the block's last value is enough
+      const hipError_t e = hipEventSynchronize(b->stage_ev[(src[j] + len[j] - 1) / kPiece]);
+      if (e != hipSuccess) {
+        int ok = (int)hipSuccess;
+        err.compare_exchange_strong(ok, (int)e);
+        return;
+      }
       const double* s = b->h_stage + src[j];
       double* d = dst[j];
       if (accumulate) {
@@ -2095,6 +2121,8 @@ int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* s
   }
   for (const int64_t t : mine) work(t);
   for (auto& th : pool) th.join();
+  HIPCHECK((hipError_t)err.load());
+  HIPCHECK(hipStreamSynchronize(b->stream));  // every piece is in before the staging buffer is reused
   return MCV_OK;
 }
 

@@ -298,8 +298,9 @@ int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n);
  * the len[j] doubles from offset src[j] of the layout mcv_batch_histograms_read
  * returns, written to dst[j] (accumulate 0) or added to it (accumulate 1).
  * One device-to-host copy into the batch's pinned staging buffer (allocated on
- * first use, freed with the batch), then the blocks are written by up to 8
- * host threads: what a scan's caller needs to fill each instance's monitor
+ * first use, freed with the batch), in 8 MB pieces, while up to 8 host
+ * threads write each block out as soon as its piece is in: what a scan's
+ * caller needs to fill each instance's monitor
  * arrays (acc/components/monitors/*.py `out`), without a temporary copy.
  * Blocks must not overlap in dst. */
 int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* src, const int64_t* len,
