@@ -113,6 +113,25 @@ def test_batch_histograms_in_one_copy(scan_and_expected):
                 np.testing.assert_array_equal(np.ascontiguousarray(c.out).reshape(3, -1), want)
 
 
+def test_batch_deliver_rejects_bad_blocks(scan_and_expected):
+    """A block reaching past the batch's histograms, or a destination the
+    runtime cannot write, is refused before anything is copied."""
+    from acc_amd._abi import McvError
+    scan, _ = scan_and_expected
+    h = scan.handle
+    n = h.ninst * 3 * sum(h.nbins)
+    dst = np.zeros(16)
+    with pytest.raises(McvError, match="outside"):
+        h.deliver_histograms([(n - 8, dst)])
+    with pytest.raises(McvError, match="outside"):
+        h.deliver_histograms([(-1, dst)])
+    with pytest.raises(McvError, match="C-contiguous"):
+        h.deliver_histograms([(0, np.zeros((4, 8))[:, ::2])])
+    assert not dst.any()
+    h.deliver_histograms([(8, dst)])  # in range: the 16 doubles from offset 8
+    np.testing.assert_array_equal(dst, h.read_histograms().reshape(-1)[8:24])
+
+
 def test_batch_sequential_matches(scan_and_expected):
     scan, exp = scan_and_expected
     n = [p[2] for p in PLAN]
