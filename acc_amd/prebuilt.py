@@ -182,6 +182,38 @@ def _storage_psd():
     return i
 
 
+def _two_samples():
+    """Source -> two drawing samples in a row -> PSD: two stochastic
+    components after the culling point share each neutron's carried stream."""
+    from .instrument import instrument
+    from .configs import _source, isotropic_al_kernel
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .geometry import Sphere, Block
+    i = instrument("two-samples")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(factory(shape=Sphere(0.02), kernel=isotropic_al_kernel())('s1'), position=(0, 0, 10.0))
+    i.append(factory(shape=Block(0.05, 0.05, 0.01), kernel=isotropic_al_kernel())('s2'), position=(0, 0, 10.2))
+    i.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=50, ny=50), position=(0, 0, 11.0))
+    return i
+
+
+def _three_guides():
+    """Source -> three straight guides with gaps -> PSD: phase A ends at the
+    first guide's entrance; the later guides run whole in phase B."""
+    from .instrument import instrument
+    from .configs import _source
+    from .components.optics.guide import Guide
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("three-guides")
+    i.append(_source(52.5, 47.5), position=(0, 0, 0))
+    for k, (w, m) in enumerate(((0.05, 2.0), (0.045, 3.0), (0.04, 1.0))):
+        i.append(Guide('guide%d' % k, w1=w, h1=w, w2=w - 0.005, h2=w - 0.005, l=1.0, m=m),
+                 position=(0, 0, 10.0 + 1.1 * k))
+    i.append(PSD_monitor('psd', xwidth=0.06, yheight=0.06, nx=30, ny=30), position=(0, 0, 13.5))
+    return i
+
+
 def _aperture_monitors():
     """Source -> slit -> circular beamstop -> PosDiv monitor -> PSD_monitor_4Pi:
     the cheap optics and monitors of the reference's sgm/sphere scripts."""
@@ -250,6 +282,8 @@ def instruments():
         "grid-sphere": grid_sphere,
         "storage-psd": _storage_psd,
         "aperture-monitors": _aperture_monitors,
+        "two-samples": _two_samples,
+        "three-guides": _three_guides,
     }
 
 

@@ -71,6 +71,10 @@ CASES = [
     ("hms-sample", True, 30000, 2048),
     ("hms-sample", False, 20000, 1024),
     ("iqe-normalizer", False, 200000, 4096),
+    ("two-samples", True, 200000, 4096),
+    ("two-samples", False, 100000, 2048),
+    ("three-guides", True, 200000, 4096),
+    ("three-guides", False, 100000, 2048),
 ]
 
 
