@@ -214,6 +214,52 @@ def _three_guides():
     return i
 
 
+def _mixed_chain():
+    """Source -> PSD -> slit -> guide -> tapered guide -> sphere -> beamstop
+    -> I(Q,E) -> wavelength monitor: a monitor and an aperture before the
+    culling point, a drawing sample after it, an aperture and two monitors
+    after the sample."""
+    from .instrument import instrument
+    from .configs import _source, _guides, isotropic_al_kernel
+    from .geometry import Sphere
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.optics.slit import Slit
+    from .components.optics.beamstop import Beamstop
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.iqe_monitor import IQE_monitor
+    from .components.monitors.wavelength_monitor import Wavelength_monitor
+    i = instrument("mixed")
+    i.append(_source(52.5, 47.5), position=(0, 0, 0))
+    i.append(PSD_monitor('psd0', xwidth=0.2, yheight=0.2, nx=20, ny=20), position=(0, 0, 9.0))
+    i.append(Slit('slit', width=0.06, height=0.06, cut=0.0), position=(0, 0, 9.9))
+    _guides(i)
+    i.append(factory(shape=Sphere(0.01), kernel=isotropic_al_kernel())('sample'), position=(0, 0, 30.5))
+    i.append(Beamstop('stop', radius=0.005), position=(0, 0, 31.0))
+    i.append(IQE_monitor('iqe', Ei=52.5, Qmin=0., Qmax=12., nQ=60, Emin=-50., Emax=50., nE=50),
+             position=(0, 0, 30.5))
+    i.append(Wavelength_monitor('wavelength', Lmin=1., Lmax=10., nchan=100, xwidth=0.2, yheight=0.2),
+             position=(0, 0, 31.5))
+    return i
+
+
+def _guide_ms():
+    """Source -> straight guide -> hollow cylinder with multiple scattering ->
+    PSD: a culling guide before the multiple scatterer."""
+    from .instrument import instrument
+    from .configs import _source, hollow_cylinder, isotropic_al_kernel
+    from .components.optics.guide import Guide
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    HMS = factory(shape=hollow_cylinder(), kernel=isotropic_al_kernel(50.0, 5.0), max_ms_loops=2,
+                  max_ms_loops_path1=3, max_scattered_neutrons=9)
+    i = instrument("guide-ms")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(Guide('guide', w1=0.05, h1=0.05, w2=0.03, h2=0.03, l=2.0), position=(0, 0, 10.0))
+    i.append(HMS('sample'), position=(0, 0, 12.5), orientation=(90, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 13.0))
+    return i
+
+
 def _aperture_monitors():
     """Source -> slit -> circular beamstop -> PosDiv monitor -> PSD_monitor_4Pi:
     the cheap optics and monitors of the reference's sgm/sphere scripts."""
@@ -284,6 +330,8 @@ def instruments():
         "aperture-monitors": _aperture_monitors,
         "two-samples": _two_samples,
         "three-guides": _three_guides,
+        "mixed": _mixed_chain,
+        "guide-ms": _guide_ms,
     }
 
 

@@ -17,6 +17,21 @@ def rel_errors(g, o):
     return rel
 
 
+def ill_conditioned(o, oa, o_nudged):
+    """How many of the oracle's neutrons with an output move by more than TOL
+    when every transcendental is moved by one ulp: the neutrons that a 1-ulp
+    difference between host and device libm can put beyond the tolerance."""
+    kappa = np.max([rel_errors(on, o).max(axis=1) for on in o_nudged], axis=0)
+    return int(((oa >= 1) & (kappa > TOL)).sum())
+
+
+def outlier_budget(n, n_ill):
+    """Conditioning outliers a run of n neutrons may show: a few per 1e4
+    neutrons, or 5 % of the ill-conditioned ones, where libm results that
+    differ in their last bit (a small fraction of the calls) reach them."""
+    return max(2, 1e-4 * n, 0.05 * n_ill)
+
+
 def compare_states(g, o, ga, oa, o_nudged=None):
     """(max error, flips, agreeing, conditioning outliers, max error of the rest).
 

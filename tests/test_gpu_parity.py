@@ -8,6 +8,10 @@ discrete path differs (a branch decided on values equal to the last ulp,
 e.g. a t2 < t1 tie) is a "flip": flips must be rare (<= 1e-4 of the
 neutrons) and are reported; with no flip, histograms agree bin by bin (counts
 exactly, sums to 1e-9 relative: only the atomic summation order differs).
+A neutron off by more than 1e-12 (but at most 1e-9) passes only as a
+conditioning outlier: the oracle itself, rerun with its transcendentals moved
+by one ulp, moves it by at least 1/8 of the deviation.  Their number is
+bounded by a few per 1e4 neutrons or 5 % of the neutrons so sensitive.
 """
 import numpy as np
 import pytest
@@ -17,7 +21,7 @@ from acc_amd import prebuilt
 from acc_amd._abi import MCV_FLAG_GATE_DEAD, rng_states
 from acc_amd.instrument import calcTransformations
 from acc_amd.run_script import get_handle
-from parity_tools import TOL, compare_states
+from parity_tools import TOL, compare_states, ill_conditioned, outlier_budget
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +79,10 @@ CASES = [
     ("two-samples", False, 100000, 2048),
     ("three-guides", True, 200000, 4096),
     ("three-guides", False, 100000, 2048),
+    ("mixed", True, 200000, 4096),
+    ("mixed", False, 100000, 2048),
+    ("guide-ms", True, 200000, 4096),
+    ("guide-ms", False, 100000, 2048),
 ]
 
 
@@ -104,10 +112,11 @@ def test_chain_parity(name, gate, N, ntot):
     finally:
         pyoracle.set_trans_nudge(0)
     err, flips, ngood, nout, err_rest = compare_states(g, o, ga, oa, on)
+    n_ill = ill_conditioned(o, oa, on)
     print(f"[parity] {name}: {ngood} alive neutrons, max err {err:.2e} ({err_rest:.2e} without "
-          f"{nout} conditioning outliers), flips {flips}", flush=True)
+          f"{nout} conditioning outliers; {n_ill} ill-conditioned), flips {flips}", flush=True)
     assert err_rest <= TOL
-    assert nout <= max(2, 1e-4 * N)
+    assert nout <= outlier_budget(N, n_ill)
     assert err <= 1e-9
     assert flips <= max(2, 1e-4 * N)
     for i, c in enumerate(instr.components):
