@@ -220,7 +220,7 @@ def copy_kernel(nbytes=1 << 31, reps=3):
 CONFIG_ROWS = (("c1", "c1", int(1e7), True, int(1e6)), ("c1-one-launch", "c1", int(1e7), True, 0),
                ("c2", "c2", int(1e8), True, 0),
                ("c2-ungated", "c2", int(1e8), False, 0), ("c2s", "c2s", int(1e8), True, 0),
-               ("c3", "c3", int(1e8), True, 0))
+               ("c2s-ungated", "c2s", int(1e8), False, 0), ("c3", "c3", int(1e8), True, 0))
 
 
 def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_neutrons=1000000):
