@@ -19,6 +19,7 @@ class NeutronFromStorage(SourceBase):
 
     requires_neutron_index_in_processing = True
     mcv_type = MCV_SRC_STORAGE
+    sets_spins = True  # stored rows carry s1, s2 (the fused kernels' queues then keep them)
 
     def __init__(self, name, path):
         """name: component name; path: neutron file (.mcv)."""

@@ -387,9 +387,19 @@ __device__ __forceinline__ void run_lane_process_ms(const LaunchArgs& A, int64_t
 // still has work; each wave publishes its two counts in a per-round slot
 // (double-buffered by round parity, waves are never more than one round
 // apart), so one barrier per round makes every lane see the same values.
+// Queued neutrons keep their spins only where a source can set them
+// (MCV_SPINS, e.g. stored neutrons); every other source leaves them 0, and
+// no component changes them, so the queue need not hold them.
+#ifndef MCV_SPINS
+#define MCV_SPINS 0
+#endif
+
 template <int CAP>
 struct QueueCore {
   double x[CAP], y[CAP], z[CAP], vx[CAP], vy[CAP], vz[CAP], t[CAP], p[CAP];
+#if MCV_SPINS
+  double s1[CAP], s2[CAP];
+#endif
   int64_t idx[CAP];
   int count;  // slot allocator
   int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
@@ -407,6 +417,9 @@ static_assert(MCV_BLOCK * MCV_SPL <= 65536, "stream slots must fit the 16-bit ow
 struct QueueRngLDS {
   double x[MCV_QCAP], y[MCV_QCAP], z[MCV_QCAP], vx[MCV_QCAP], vy[MCV_QCAP], vz[MCV_QCAP], t[MCV_QCAP],
       p[MCV_QCAP];
+#if MCV_SPINS
+  double s1[MCV_QCAP], s2[MCV_QCAP];
+#endif
   // narrow fields keep a 128-lane, 2-slot workgroup's queue under a sixth of the
   // CU's LDS, so LDS does not cap residency below the register limit
   uint16_t owner[MCV_QCAP];  // stream slot: lane * MCV_SPL + stream of the lane
@@ -417,26 +430,35 @@ struct QueueRngLDS {
   int wave_new[2][MCV_BLOCK / 64], wave_more[2][MCV_BLOCK / 64];
 };
 
-__device__ __forceinline__ void q_store_state(QueueRngLDS& q, int s, const Neutron& n) {
+template <class Q>
+__device__ __forceinline__ void q_store_state(Q& q, int s, const Neutron& n) {
   q.x[s] = n.x; q.y[s] = n.y; q.z[s] = n.z; q.vx[s] = n.vx; q.vy[s] = n.vy; q.vz[s] = n.vz;
   q.t[s] = n.t; q.p[s] = n.p;
+#if MCV_SPINS
+  q.s1[s] = n.s1; q.s2[s] = n.s2;
+#endif
 }
 
-__device__ __forceinline__ void q_load_state(const QueueRngLDS& q, int s, Neutron& n) {
+template <class Q>
+__device__ __forceinline__ void q_load_state(const Q& q, int s, Neutron& n) {
   n.x = q.x[s]; n.y = q.y[s]; n.z = q.z[s]; n.vx = q.vx[s]; n.vy = q.vy[s]; n.vz = q.vz[s];
-  n.t = q.t[s]; n.p = q.p[s]; n.s1 = 0; n.s2 = 0;
+  n.t = q.t[s]; n.p = q.p[s];
+#if MCV_SPINS
+  n.s1 = q.s1[s]; n.s2 = q.s2[s];
+#else
+  n.s1 = 0; n.s2 = 0;
+#endif
 }
 
 template <class Q>
 __device__ __forceinline__ void q_store(Q& q, int s, const Neutron& n, int64_t idx) {
-  q.x[s] = n.x; q.y[s] = n.y; q.z[s] = n.z; q.vx[s] = n.vx; q.vy[s] = n.vy; q.vz[s] = n.vz;
-  q.t[s] = n.t; q.p[s] = n.p; q.idx[s] = idx;
+  q_store_state(q, s, n);
+  q.idx[s] = idx;
 }
 
 template <class Q>
 __device__ __forceinline__ int64_t q_load(const Q& q, int s, Neutron& n) {
-  n.x = q.x[s]; n.y = q.y[s]; n.z = q.z[s]; n.vx = q.vx[s]; n.vy = q.vy[s]; n.vz = q.vz[s];
-  n.t = q.t[s]; n.p = q.p[s]; n.s1 = 0; n.s2 = 0;
+  q_load_state(q, s, n);
   return q.idx[s];
 }
 

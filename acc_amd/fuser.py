@@ -149,6 +149,8 @@ def generate_source(instrument, compact=True, block=None, waves_per_eu=None, str
             raise ValueError("waves per SIMD must be in [1, 8], got %r" % wpe)
         defs += "#define MCV_WAVES_PER_EU %d\n" % int(wpe)
     src, structure = _generate(instrument, compact)
+    if any(getattr(c, "sets_spins", False) for c in instrument.components):
+        defs += "#define MCV_SPINS 1\n"  # queued neutrons keep their spins
     if "MCV_DEFINE_KERNELS_PLAIN(" in src:
         defs += "#define MCV_LDS_CACHE 1\n"
     if "MCV_DEFINE_KERNELS_COMPACTED_RNG" in src:

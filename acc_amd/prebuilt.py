@@ -182,6 +182,53 @@ def _storage_psd():
     return i
 
 
+def spin_beam_path():
+    """A stored beam of 4096 neutrons with non-zero spins (seeded, written
+    once into the kernel cache directory): x, y within 2 cm of the axis,
+    directions within 1 % of z, speeds 2-4 km/s, s1 and s2 in (-1, 1)."""
+    import numpy as np
+    from .fuser import CACHE_DIR
+    from .neutron_storage import write
+    path = os.path.join(CACHE_DIR, "spin_beam_4096.mcv")
+    if not os.path.exists(path):
+        g = np.random.default_rng(4096)
+        n = 4096
+        rows = np.zeros((n, 10))
+        rows[:, 0:2] = g.uniform(-0.02, 0.02, (n, 2))
+        v = g.uniform(2000.0, 4000.0, n)
+        rows[:, 3:5] = g.uniform(-0.01, 0.01, (n, 2)) * v[:, None]
+        rows[:, 5] = v
+        rows[:, 6:8] = g.uniform(-1.0, 1.0, (n, 2))
+        rows[:, 9] = 1.0
+        os.makedirs(CACHE_DIR, exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        write(tmp, rows)
+        os.replace(tmp, path)
+    return path
+
+
+def _storage_guide(sample):
+    """Stored neutrons with spins -> straight guide -> (sphere, or a second
+    guide) -> PSD: the spins go through the compacted kernel's queue (the
+    stream-carrying one with the sphere, the plain one without)."""
+    from .instrument import instrument
+    from .configs import isotropic_al_kernel
+    from .components.sources.neutronfromstorage import NeutronFromStorage
+    from .components.optics.guide import Guide
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .geometry import Sphere
+    i = instrument("storage-guide" + ("-sample" if sample else ""))
+    i.append(NeutronFromStorage('src', spin_beam_path()), position=(0, 0, 0))
+    i.append(Guide('guide', w1=0.03, h1=0.03, w2=0.03, h2=0.03, l=2.0), position=(0, 0, 1.0))
+    if sample:
+        i.append(factory(shape=Sphere(0.02), kernel=isotropic_al_kernel())('sample'), position=(0, 0, 3.2))
+    else:
+        i.append(Guide('guide2', w1=0.03, h1=0.03, w2=0.02, h2=0.02, l=0.3), position=(0, 0, 3.1))
+    i.append(PSD_monitor('psd', xwidth=0.2, yheight=0.2, nx=20, ny=20), position=(0, 0, 3.5))
+    return i
+
+
 def _two_samples():
     """Source -> two drawing samples in a row -> PSD: two stochastic
     components after the culling point share each neutron's carried stream."""
@@ -331,6 +378,8 @@ def instruments():
         "two-samples": _two_samples,
         "three-guides": _three_guides,
         "mixed": _mixed_chain,
+        "storage-guide": lambda: _storage_guide(False),
+        "storage-guide-sample": lambda: _storage_guide(True),
         "guide-ms": _guide_ms,
     }
 

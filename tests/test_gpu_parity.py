@@ -83,6 +83,10 @@ CASES = [
     ("mixed", False, 100000, 2048),
     ("guide-ms", True, 200000, 4096),
     ("guide-ms", False, 100000, 2048),
+    # stored neutrons with spins through the queue (plain and stream-carrying)
+    ("storage-guide", True, 50000, 4096),
+    ("storage-guide-sample", True, 50000, 4096),
+    ("storage-guide-sample", False, 50000, 2048),
 ]
 
 
