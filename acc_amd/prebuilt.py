@@ -229,6 +229,42 @@ def _storage_guide(sample):
     return i
 
 
+def _rotated():
+    """Source -> arm turned 0.4 deg about y and 0.2 deg about x -> guide
+    placed relative to the arm -> PSD tilted by 10 deg, relative to the arm:
+    multi-axis orientations and positions relative to another component."""
+    from .instrument import instrument
+    from .configs import _source
+    from .components.optics.arm import Arm
+    from .components.optics.guide import Guide
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("rotated")
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    arm = Arm('arm')
+    i.append(arm, position=(0.01, 0.0, 9.5), orientation=(0.2, 0.4, 0))
+    i.append(Guide('guide', w1=0.08, h1=0.08, w2=0.06, h2=0.06, l=1.0), position=(0, 0, 0.5), relativeTo=arm)
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 2.0), orientation=(0, 10, 0),
+             relativeTo=arm)
+    return i
+
+
+def _storage_ms():
+    """Stored neutrons with spins -> multiple scatterer -> PSD: the outputs
+    keep the incident neutron's spins."""
+    from .instrument import instrument
+    from .configs import hollow_cylinder, isotropic_al_kernel
+    from .components.sources.neutronfromstorage import NeutronFromStorage
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    HMS = factory(shape=hollow_cylinder(), kernel=isotropic_al_kernel(50.0, 5.0), max_ms_loops=2,
+                  max_ms_loops_path1=2, max_scattered_neutrons=7)
+    i = instrument("storage-ms")
+    i.append(NeutronFromStorage('src', spin_beam_path()), position=(0, 0, 0))
+    i.append(HMS('sample'), position=(0, 0, 1.0), orientation=(90, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 1.5))
+    return i
+
+
 def _two_samples():
     """Source -> two drawing samples in a row -> PSD: two stochastic
     components after the culling point share each neutron's carried stream."""
@@ -378,6 +414,8 @@ def instruments():
         "two-samples": _two_samples,
         "three-guides": _three_guides,
         "mixed": _mixed_chain,
+        "rotated": _rotated,
+        "storage-ms": _storage_ms,
         "storage-guide": lambda: _storage_guide(False),
         "storage-guide-sample": lambda: _storage_guide(True),
         "guide-ms": _guide_ms,

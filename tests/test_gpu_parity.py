@@ -87,6 +87,9 @@ CASES = [
     ("storage-guide", True, 50000, 4096),
     ("storage-guide-sample", True, 50000, 4096),
     ("storage-guide-sample", False, 50000, 2048),
+    ("storage-ms", True, 50000, 4096),
+    ("rotated", True, 200000, 4096),
+    ("rotated", False, 100000, 2048),
 ]
 
 
