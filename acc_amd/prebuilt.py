@@ -265,6 +265,22 @@ def _storage_ms():
     return i
 
 
+def _taper_coatings(mx, my):
+    """Source -> 100-segment tapered guide with coating indices mx, my -> PSD:
+    m = 0 (a wall hit ends the segment, the reference's extra channel shift)
+    and m < 0 (m scaled by the entrance width over the segment's)."""
+    from .instrument import instrument
+    from .configs import _source, taper_file
+    from .components.optics.guide_tapering import Guide as TaperedGuide
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("taper-m%g-%g" % (mx, my))
+    i.append(_source(52.5, 47.5), position=(0, 0, 0))
+    i.append(TaperedGuide('taper', option="file=" + taper_file(), l=10., mx=mx, my=my, R0=0.99, Qcx=0.0219,
+                          Qcy=0.0219, alphax=6.07, alphay=6.07, W=0.003), position=(0, 0, 10.0))
+    i.append(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=50, ny=50), position=(0, 0, 20.5))
+    return i
+
+
 def _two_samples():
     """Source -> two drawing samples in a row -> PSD: two stochastic
     components after the culling point share each neutron's carried stream."""
@@ -414,6 +430,8 @@ def instruments():
         "two-samples": _two_samples,
         "three-guides": _three_guides,
         "mixed": _mixed_chain,
+        "taper-m0": lambda: _taper_coatings(0.0, -2.0),
+        "taper-mneg": lambda: _taper_coatings(-1.5, 0.0),
         "rotated": _rotated,
         "storage-ms": _storage_ms,
         "storage-guide": lambda: _storage_guide(False),

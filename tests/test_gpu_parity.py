@@ -90,6 +90,9 @@ CASES = [
     ("storage-ms", True, 50000, 4096),
     ("rotated", True, 200000, 4096),
     ("rotated", False, 100000, 2048),
+    ("taper-m0", True, 200000, 4096),
+    ("taper-m0", False, 100000, 2048),
+    ("taper-mneg", True, 200000, 4096),
 ]
 
 
