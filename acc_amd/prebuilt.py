@@ -281,6 +281,24 @@ def _taper_coatings(mx, my):
     return i
 
 
+def _hms_shape(kind):
+    """Source -> multiple scatterer on a solid sphere or a plate (the
+    scatterer's geometry on convex shapes: one entry, one exit) -> PSD."""
+    from .instrument import instrument
+    from .configs import _source, isotropic_al_kernel
+    from .geometry import Sphere, Block
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    shape = Sphere(0.02) if kind == "sphere" else Block(0.04, 0.04, 0.01)
+    HMS = factory(shape=shape, kernel=isotropic_al_kernel(80.0, 5.0), max_ms_loops=3, max_ms_loops_path1=2,
+                  max_scattered_neutrons=10)
+    i = instrument("hms-" + kind)
+    i.append(_source(60.0, 10.0), position=(0, 0, 0))
+    i.append(HMS('sample'), position=(0, 0, 10.0), orientation=(0, 30, 0))
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 10.5))
+    return i
+
+
 def _two_samples():
     """Source -> two drawing samples in a row -> PSD: two stochastic
     components after the culling point share each neutron's carried stream."""
@@ -430,6 +448,8 @@ def instruments():
         "two-samples": _two_samples,
         "three-guides": _three_guides,
         "mixed": _mixed_chain,
+        "hms-sphere": lambda: _hms_shape("sphere"),
+        "hms-plate": lambda: _hms_shape("plate"),
         "taper-m0": lambda: _taper_coatings(0.0, -2.0),
         "taper-mneg": lambda: _taper_coatings(-1.5, 0.0),
         "rotated": _rotated,

@@ -25,6 +25,30 @@ def ill_conditioned(o, oa, o_nudged):
     return int(((oa >= 1) & (kappa > TOL)).sum())
 
 
+def nudge_flips(o, oa, nudged):
+    """How many neutrons the oracle itself sends down another discrete path
+    (other output count, or an O(1) state change) when its transcendentals
+    are moved by one ulp; `nudged` holds (states, output counts) per rerun.
+    Chains whose branch decisions hang on the last bit of a position (the
+    reference's box locate() has no margin for `inside`, so a neutron moved
+    onto a face lands inside or on it by rounding) show many; for most chains
+    there are none."""
+    m = np.zeros(len(o), dtype=bool)
+    for on, ona in nudged:
+        rel = rel_errors(on, o).max(axis=1)
+        rel[(oa == 0) & (ona == 0)] = 0
+        m |= (ona != oa) | (rel > 1e-6)
+    return int(m.sum())
+
+
+def flip_budget(n, n_nudge_flips):
+    """Discrete path differences a run of n neutrons may show against the
+    oracle: a few per 1e4 neutrons, or a tenth of those that 1-ulp libm
+    perturbations cause in the oracle itself (host and device libm differ in
+    the last bit on a small fraction of calls)."""
+    return max(2, 1e-4 * n, 0.1 * n_nudge_flips)
+
+
 def outlier_budget(n, n_ill):
     """Conditioning outliers a run of n neutrons may show: a few per 1e4
     neutrons, or 5 % of the ill-conditioned ones, where libm results that
