@@ -299,6 +299,58 @@ def _hms_shape(kind):
     return i
 
 
+def batch_plain(k):
+    """Instance k of a batch of plain chains: source (E0 40 + 15 k meV) ->
+    sphere (r 8 + 2 k mm) -> I(Q,E) -> PSD 250 x 250 (the LDS bin cache)."""
+    from .instrument import instrument
+    from .configs import _source, isotropic_al_kernel
+    from .geometry import Sphere
+    from .components.samples.homogeneous_single_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.iqe_monitor import IQE_monitor
+    E0 = 40.0 + 15.0 * k
+    i = instrument("batch-plain-%d" % k)
+    i.append(_source(E0, 8.0), position=(0, 0, 0))
+    sample = factory(shape=Sphere(0.008 + 0.002 * k), kernel=isotropic_al_kernel())('sample')
+    i.append(sample, position=(0, 0, 10.0))
+    i.append(IQE_monitor('iqe', Ei=E0, Qmin=0., Qmax=12., nQ=120, Emin=-50., Emax=50., nE=100),
+             position=(0, 0, 0), relativeTo=sample)
+    i.append(PSD_monitor('psd', xwidth=0.5, yheight=0.5, nx=250, ny=250), position=(0, 0, 11.0))
+    return i
+
+
+def batch_compacted(k):
+    """Instance k of a batch of compacted chains that draw nothing after the
+    culling point: source (E0 30 + 20 k meV) -> guides -> wavelength + PSD."""
+    from .instrument import instrument
+    from .configs import _source, _guides
+    from .components.monitors.psd_monitor import PSD_monitor
+    from .components.monitors.wavelength_monitor import Wavelength_monitor
+    i = instrument("batch-compacted-%d" % k)
+    i.append(_source(30.0 + 20.0 * k, 10.0), position=(0, 0, 0))
+    _guides(i)
+    i.append(Wavelength_monitor('wavelength', Lmin=1., Lmax=10., nchan=200, xwidth=0.06, yheight=0.06),
+             position=(0, 0, 30.001))
+    i.append(PSD_monitor('psd', xwidth=0.06, yheight=0.06, nx=100, ny=100), position=(0, 0, 30.001))
+    return i
+
+
+def batch_ms(k):
+    """Instance k of a batch of multiple-scattering chains: source (E0 50 +
+    10 k meV) -> hollow cylinder (scattering coefficient 40 + 20 k /m) -> PSD."""
+    from .instrument import instrument
+    from .configs import _source, hollow_cylinder, isotropic_al_kernel
+    from .components.samples.homogeneous_multiple_scatterer import factory
+    from .components.monitors.psd_monitor import PSD_monitor
+    HMS = factory(shape=hollow_cylinder(), kernel=isotropic_al_kernel(40.0 + 20.0 * k, 5.0), max_ms_loops=2,
+                  max_ms_loops_path1=2, max_scattered_neutrons=7)
+    i = instrument("batch-ms-%d" % k)
+    i.append(_source(50.0 + 10.0 * k, 10.0), position=(0, 0, 0))
+    i.append(HMS('sample'), position=(0, 0, 10.0), orientation=(90, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.4, yheight=0.4, nx=40, ny=40), position=(0, 0, 10.5))
+    return i
+
+
 def _two_samples():
     """Source -> two drawing samples in a row -> PSD: two stochastic
     components after the culling point share each neutron's carried stream."""
@@ -448,6 +500,9 @@ def instruments():
         "two-samples": _two_samples,
         "three-guides": _three_guides,
         "mixed": _mixed_chain,
+        "batch-plain": lambda: batch_plain(0),
+        "batch-compacted": lambda: batch_compacted(0),
+        "batch-ms": lambda: batch_ms(0),
         "hms-sphere": lambda: _hms_shape("sphere"),
         "hms-plate": lambda: _hms_shape("plate"),
         "taper-m0": lambda: _taper_coatings(0.0, -2.0),
