@@ -207,6 +207,18 @@ def spin_beam_path():
     return path
 
 
+def _storage_beam_psd():
+    """The stored 4096-neutron beam with spins -> PSD (a plain kernel whose
+    source reads row i % 4096 for neutron i)."""
+    from .instrument import instrument
+    from .components.sources.neutronfromstorage import NeutronFromStorage
+    from .components.monitors.psd_monitor import PSD_monitor
+    i = instrument("storage-beam-psd")
+    i.append(NeutronFromStorage('src', spin_beam_path()), position=(0, 0, 0))
+    i.append(PSD_monitor('psd', xwidth=0.1, yheight=0.1, nx=50, ny=50), position=(0, 0, 1.0))
+    return i
+
+
 def _storage_guide(sample):
     """Stored neutrons with spins -> straight guide -> (sphere, or a second
     guide) -> PSD: the spins go through the compacted kernel's queue (the
@@ -509,6 +521,7 @@ def instruments():
         "taper-mneg": lambda: _taper_coatings(-1.5, 0.0),
         "rotated": _rotated,
         "storage-ms": _storage_ms,
+        "storage-beam-psd": _storage_beam_psd,
         "storage-guide": lambda: _storage_guide(False),
         "storage-guide-sample": lambda: _storage_guide(True),
         "guide-ms": _guide_ms,

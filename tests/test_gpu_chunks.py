@@ -42,11 +42,18 @@ def _same(a, b):
         np.testing.assert_allclose(h1[i][1:], h0[i][1:], rtol=1e-12, atol=1e-300)
 
 
-# plain (c1: slices of every stream; the last case's slice is one neutron),
-# compacted (c2), stream-carrying pooled (c2s), multiple scattering (c3-small)
+# plain (c1: slices of every stream; the last case's slice is one neutron;
+# stored neutrons, whose row is the neutron's index), compacted (c2),
+# stream-carrying pooled (c2s; stored neutrons with spins), multiple
+# scattering (c3-small)
+PLAIN = ("c1", "storage-beam-psd")
+
+
 @pytest.mark.parametrize("name,N,ntot,buf", [("c1", 400000, 20000, 57000), ("c1", 399999, 20000, 1000),
                                              ("c2", 300000, 20000, 61000), ("c2s", 300000, 20000, 45000),
-                                             ("c3-small", 60000, 4096, 9000), ("c2", 300000, 20000, 7)])
+                                             ("c3-small", 60000, 4096, 9000), ("c2", 300000, 20000, 7),
+                                             ("storage-beam-psd", 100003, 4096, 9000),
+                                             ("storage-guide-sample", 60000, 4096, 7000)])
 def test_chunked_run_equals_one_launch(name, N, ntot, buf):
     print(f"\n[chunks] {name} N={N} ntot={ntot} buffer_size={buf}", flush=True)
     instr = prebuilt.instruments()[name]()
@@ -56,7 +63,7 @@ def test_chunked_run_equals_one_launch(name, N, ntot, buf):
     npt = one[0]["npt"]
     streams = -(-N // npt)
     assert one[0]["nchunks"] == 1
-    if name == "c1":  # slices of max(1, buf // streams) neutrons of every stream
+    if name in PLAIN:  # slices of max(1, buf // streams) neutrons of every stream
         per = max(1, buf // streams)
         assert many[0]["nchunks"] == -(-npt // per), many[0]
     else:  # ranges of max(1, buf // npt) whole streams
