@@ -150,3 +150,47 @@ def test_chain_parity(name, gate, N, ntot):
         else:  # a flipped neutron changes at most its own outputs' counts, on either side
             assert abs(gh[0].sum() - oh[0].sum()) <= 2 * flips * max(1, int(oa.max()), int(ga.max()))
         assert oh[0].sum() > 0, f"{c.name} saw no neutron: test chain is not exercising it"
+
+
+# Partition edges: one neutron, fewer neutrons than threads, workgroup sizes
+# (the reference's threads_per_block) that are not powers of two, the last
+# stream short, ntotalthreads below the workgroup size.
+PARTITIONS = [(1, 1000000, 512), (7, 1000000, 100), (1000, 333, 128), (12345, 4096, 1000), (100000, 99999, 77),
+              (5000, 50, 64)]
+
+
+@pytest.mark.parametrize("name", ["c1", "c2s", "c3-small"])
+@pytest.mark.parametrize("N,ntot,tpb", PARTITIONS)
+def test_partition_edges(name, N, ntot, tpb):
+    print(f"\n[partition] {name} N={N} ntot={ntot} tpb={tpb}", flush=True)
+    instr = prebuilt.instruments()[name]()
+    h = get_handle(instr)
+    h.set_dump(0, N)
+    h.clear()
+    stats = h.run(N, 2, ntot, tpb, MCV_FLAG_GATE_DEAD)
+    assert stats["processed"] == N
+    g, ga = h.read_dump()
+    descs = [c.mcv_desc() for c in instr.components]
+    off, rot = calcTransformations(instr)
+    hists, processed, o, oa = pyoracle.run(descs, off, rot, N, 2, ntotalthreads=ntot, tpb=tpb, gate=True,
+                                           dump=(0, N))
+    assert processed == N
+    nblocks = -(-min(N, ntot) // tpb)
+    assert stats["nblocks"] == nblocks and stats["nthreads"] == nblocks * tpb
+    assert stats["npt"] == -(-N // (nblocks * tpb))
+    nudged = []
+    try:
+        for mode in (1, -1, 2):
+            pyoracle.set_trans_nudge(mode)
+            nudged.append(pyoracle.run(descs, off, rot, N, 2, ntotalthreads=ntot, tpb=tpb, gate=True,
+                                       dump=(0, N))[2])
+    finally:
+        pyoracle.set_trans_nudge(0)
+    err, flips, ngood, nout, err_rest = compare_states(g, o, ga, oa, nudged)
+    assert flips == 0 and err_rest <= TOL and err <= 1e-9, (err, err_rest, flips)
+    assert nout <= outlier_budget(N, ill_conditioned(o, oa, nudged))
+    for i, c in enumerate(instr.components):
+        if hists[i] is not None:
+            gh = h.read_monitor(i)
+            assert np.array_equal(gh[0], hists[i][0]), c.name
+            np.testing.assert_allclose(gh[1:], hists[i][1:], rtol=1e-9, atol=1e-300, err_msg=c.name)
