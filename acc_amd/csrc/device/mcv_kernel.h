@@ -378,10 +378,13 @@ __device__ __forceinline__ void run_lane_process_ms(const LaunchArgs& A, int64_t
 // prefix count, one LDS atomic per wave), and phase B runs over the queue with
 // all lanes of whole wavefronts busy instead of one in four.
 //
-// Chain::phase_a returns r > 0 to queue the neutron; r <= 0 when it is done in
-// phase A, with -r its dump flag (0 for a dropped neutron; a multiple-
-// scattering chain whose neutron misses the sample finishes its one output in
-// phase A and returns -1).  Chain::phase_b returns the dump flag.
+// Chain::phase_a returns r > 0 to queue the neutron (2: phase B is to skip
+// the rest of the component phase A ended in, which did not let the neutron
+// in); r <= 0 when it is done in phase A, with -r its dump flag (0 for a
+// dropped neutron; a multiple-scattering chain whose neutron misses the
+// sample finishes its one output in phase A and returns -1).
+// Chain::phase_b(..., skip) returns the dump flag; the queue keeps `skip` in
+// the sign of the neutron index or the top bit of the owner slot.
 //
 // Control decisions need the workgroup-wide queue length and whether any lane
 // still has work; each wave publishes its two counts in a per-round slot
@@ -408,7 +411,8 @@ struct QueueCore {
 // deterministic phase B: up to (MCV_BLOCK - 1) + MCV_BLOCK queued neutrons
 struct QueueLDS : QueueCore<MCV_QCAP> {};
 
-static_assert(MCV_BLOCK * MCV_SPL <= 65536, "stream slots must fit the 16-bit owner field");
+static_assert(MCV_BLOCK * MCV_SPL <= 32768, "stream slots must fit the owner field below its skip bit");
+constexpr uint16_t kOwnerSkip = 0x8000;
 
 // drawing phase B: the states and next neutron indices of all the
 // workgroup's streams live here.  A queued neutron names its stream slot;
@@ -516,14 +520,15 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
     n.s1 = 0;
     n.s2 = 0;
     const int64_t my = idx;
+    int r = 0;
     if (idx < end) {
-      const int r = Chain::phase_a(rng, n, A, my);
+      r = Chain::phase_a(rng, n, A, my);
       has = r > 0;
       if (!has) dump_neutron(A, my, n, -r);
       ++idx;
     }
     const int s = q_append(q, has);
-    if (has) q_store(q, s, n, my);
+    if (has) q_store(q, s, n, r > 1 ? ~my : my);
     int n_new;
     bool more;
     round_totals(q, par, has, idx < end, n_new, more);
@@ -533,8 +538,9 @@ __device__ __forceinline__ void run_block_compacted(const LaunchArgs& A, int64_t
       if (lt < take) {
         Neutron b;
         const int64_t bidx = q_load(q, cnt - take + lt, b);
-        const int alive = (int)Chain::phase_b(b, A);
-        dump_neutron(A, bidx, b, alive);
+        const bool skip = bidx < 0;
+        const int alive = (int)Chain::phase_b(b, A, skip);
+        dump_neutron(A, skip ? ~bidx : bidx, b, alive);
       }
       cnt -= take;
       __syncthreads();
@@ -648,7 +654,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     n.s2 = 0;
     Rng w;
     int64_t my = 0;
-    int sel = -1;
+    int sel = -1, ra = 0;
     if (avail) {
       const unsigned rot = ((avail >> rr) | (avail << (MCV_SPL - rr))) & ALL;
       sel = (rr + __builtin_ctz(rot)) % MCV_SPL;
@@ -656,9 +662,9 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
       w.s0 = q.st0[o];
       w.s1 = q.st1[o];
       my = q.next[o];
-      const int r = Chain::phase_a(w, n, A, my);
-      has = r > 0;
-      if (!has) dump_neutron(A, my, n, -r);
+      ra = Chain::phase_a(w, n, A, my);
+      has = ra > 0;
+      if (!has) dump_neutron(A, my, n, -ra);
       q.st0[o] = w.s0;
       q.st1[o] = w.s1;
       q.next[o] = my + 1;
@@ -669,7 +675,7 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
     const int s = q_append(q, has);
     if (has) {
       q_store_state(q, s, n);
-      q.owner[s] = lt * MCV_SPL + sel;
+      q.owner[s] = (uint16_t)(lt * MCV_SPL + sel) | (ra > 1 ? kOwnerSkip : 0);
     }
     int n_new;
     bool can_a;
@@ -681,11 +687,12 @@ __device__ __forceinline__ void run_block_compacted_rng(const LaunchArgs& A, int
         Neutron b;
         const int slot = cnt - take + lt;
         q_load_state(q, slot, b);
-        const int o = q.owner[slot];
+        const int ow = q.owner[slot];
+        const int o = ow & (kOwnerSkip - 1);
         Rng r;
         r.s0 = q.st0[o];
         r.s1 = q.st1[o];
-        const int alive = (int)Chain::phase_b(r, b, A);
+        const int alive = (int)Chain::phase_b(r, b, A, (ow & kOwnerSkip) != 0);
         dump_neutron(A, q.next[o] - 1, b, alive);
         q.st0[o] = r.s0;  // the stream continues from here
         q.st1[o] = r.s1;

@@ -164,10 +164,11 @@ def _ungated_opening(comps, k):
     """Statements ending phase A of an ungated run for a neutron that
     component k + 1 turns away at its opening: that component's reject (the
     only change it would make), then the rest of the chain, in phase A; an
-    admitted neutron is queued unchanged (phase B applies the frame change
-    and runs component k + 1 in full).  Without gating every neutron runs
-    through every component, so phase B would otherwise hold every neutron,
-    most of which the next component's opening stops at once."""
+    admitted neutron falls through to the caller's queueing (phase B applies
+    the frame change and runs component k + 1 in full).  Without gating every
+    neutron runs through every component, so phase B would otherwise hold
+    every neutron, most of which the next component's opening stops at
+    once."""
     if k + 1 >= len(comps) or comps[k + 1].propagate.opening is None:
         return ""
     nxt = comps[k + 1]
@@ -186,26 +187,28 @@ def _ungated_opening(comps, k):
 
 
 def _split_phases(comps, k):
-    """Phases A and B when component k can stop after its entrance: with
-    gating (dead neutrons dropped), phase A ends with the entrance and phase B
-    starts with the rest of component k, so that its loop runs with whole
-    wavefronts of neutrons that entered; without gating (every neutron runs
-    through every component, absorbed or not) phase A runs all of component k
-    as before, since a neutron absorbed at the entrance must then not run the
-    rest, and then, when component k + 1 has an opening test, queues only the
-    neutrons that test admits."""
+    """Phases A and B when component k can stop after its entrance: phase A
+    ends with the entrance and phase B starts with the rest of component k,
+    so that its loop runs with whole wavefronts of neutrons that entered.
+    With gating (dead neutrons dropped) a neutron the entrance absorbs is
+    dropped.  Without gating (every neutron runs through every component,
+    absorbed or not) it runs on from component k + 1: when that component
+    has an opening test and turns it away, in phase A; otherwise it is queued
+    with the skip flag (phase_a returns 2), and phase B skips the rest of
+    component k for it."""
     c = comps[k]
+    enter = c.propagate.render_split(c, k, 0)
     a = _phase_lines(comps, 0, k, False)
     if k > 0:
         a.append(f"mcv::apply_transformation(n, P + A.xoff[{k - 1}]);")
     a.append(f"""if (A.gate) {{
-    if (!{c.propagate.render_split(c, k, 0)}) return 0;
-  }} else {{
-    {c.propagate.render(c, k)}{_ungated_opening(comps, k)}
+    if (!{enter}) return 0;
+  }} else if (!{enter}) {{  // absorbed at the entrance: the rest of the chain runs on it{_ungated_opening(comps, k)}
+    return 2;  // phase B skips the rest of component {k}
   }}""")
     a.append("if (A.gate && !(n.p > 0)) return 0;")
     a.append("return 1;")
-    b = [f"if (A.gate) {{ {c.propagate.render_split(c, k, 1)} }}",
+    b = [f"if (!skip) {{ {c.propagate.render_split(c, k, 1)} }}",
          "if (A.gate && !(n.p > 0)) return 0;",
          _phase(comps, k + 1, len(comps), True)]
     return "\n  ".join(a), "\n  ".join(b)
@@ -247,10 +250,10 @@ struct Chain {{
   {a}
   }}
 
-  __device__ static __forceinline__ int phase_b({b_sig}, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int phase_b({b_sig}, const mcv::LaunchArgs& A, const bool skip) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  {b_void}
+  {b_void} (void)skip;
   {b}
   }}
 }};

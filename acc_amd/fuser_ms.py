@@ -112,10 +112,11 @@ struct Chain {{
   {a_text}
   }}
 
-  __device__ static __forceinline__ int phase_b(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A) {{
+  __device__ static __forceinline__ int phase_b(mcv::Rng& rng, mcv::Neutron& n, const mcv::LaunchArgs& A,
+                                                 const bool skip) {{
   const double* __restrict__ P = A.P;
   double* __restrict__ H = A.H;
-  (void)H;
+  (void)H; (void)skip;
   {b_text}
   }}
 }};

@@ -53,18 +53,26 @@ def _body(src, fn):
 
 
 def test_entrance_split():
-    """With gating, phase A ends at the straight guide's entrance and phase B
-    starts with its bounces; without gating phase A runs the whole guide (an
-    absorbed neutron must not run the rest).  The adjoint kernel splits the
-    same way, the guide's scores starting in phase B."""
+    """Phase A ends at the straight guide's entrance and phase B starts with
+    its bounces.  With gating an absorbed neutron is dropped; without, it runs
+    on from the next component, in phase A when that component's opening
+    turns it away, else queued with the skip flag (phase_a returns 2), phase B
+    then skipping the guide's bounces for it.  The adjoint kernel (gated
+    runs) splits the same way, the guide's scores starting in phase B; its
+    ungated phase A runs the whole guide."""
     from acc_amd.fuser_adj import generate_adjoint_source
     from acc_amd import configs
-    for src in (fuser.generate_source(prebuilt.instruments()["c2"]())[0],
-                generate_adjoint_source(configs.c5())[0]):
-        a, b = _body(src, "phase_a"), _body(src, "phase_b")
-        assert "if (A.gate) {\n    if (!mcv::guide_enter(n, P + A.off[1])) return 0;\n  } else {" in a
-        assert "mcv::guide_bounces" not in a and "mcv::guide_enter" not in b
-        assert b.index("if (A.gate) { mcv::guide_bounces") < b.index("mcv::guide_tapering")
+    src = fuser.generate_source(prebuilt.instruments()["c2"]())[0]
+    a, b = _body(src, "phase_a"), _body(src, "phase_b")
+    assert "if (A.gate) {\n    if (!mcv::guide_enter(n, P + A.off[1])) return 0;\n  } else if (!mcv::guide_enter(" in a
+    assert "mcv::taper_admits(m," in a and "return 2;" in a
+    assert "mcv::guide_bounces" not in a and "mcv::guide_enter" not in b
+    assert b.index("if (!skip) { mcv::guide_bounces") < b.index("mcv::guide_tapering")
+    src = generate_adjoint_source(configs.c5())[0]
+    a, b = _body(src, "phase_a"), _body(src, "phase_b")
+    assert "if (A.gate) {\n    if (!mcv::guide_enter(n, P + A.off[1])) return 0;\n  } else {" in a
+    assert "mcv::guide_enter" not in b
+    assert b.index("if (A.gate) { mcv::guide_bounces") < b.index("mcv::guide_tapering")
 
 
 def test_ms_outputs_streamed():
