@@ -301,7 +301,7 @@ int mcv_batch_histograms_read(mcv_batch* b, double* out, int64_t n);
  * first use, freed with the batch), in 8 MB pieces, while up to 8 host
  * threads write each block out as soon as its piece is in: what a scan's
  * caller needs to fill each instance's monitor
- * arrays (acc/components/monitors/*.py `out`), without a temporary copy.
+ * arrays (the `out` arrays of acc/components/monitors), without a temporary copy.
  * Blocks must not overlap in dst. */
 int mcv_batch_histograms_deliver(mcv_batch* b, int64_t nblocks, const int64_t* src, const int64_t* len,
                                  double* const* dst, int32_t accumulate);

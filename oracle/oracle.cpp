@@ -56,11 +56,25 @@ namespace orc {
 // a 1-ulp difference between two correct libm implementations (the host's and
 // the GPU's) legitimately shows up in its final state.
 static int g_nudge = 0;
+// Reproduction hook (test hook): for the one neutron g_pin_idx, the k-th
+// transcendental result of its history (counted from 0 at its start) is moved
+// by g_pin_dir[k] ulps (-1, 0, +1); every other neutron is left alone.  A
+// device state that this reproduces bit for bit differs from the oracle's by
+// exactly those libm ulps.
+static int64_t g_pin_idx = -1;
+static std::vector<int> g_pin_dir;
 #ifdef ORC_FLOP_BUILD
 template <class T>
 static inline T tr(T v) { return v; }
 #else
+static thread_local bool t_pin = false;
+static thread_local size_t t_call = 0;
 static inline double tr(double v) {
+  if (t_pin) {
+    const size_t k = t_call++;
+    if (k < g_pin_dir.size() && g_pin_dir[k]) v = std::nextafter(v, g_pin_dir[k] > 0 ? INFINITY : -INFINITY);
+    return v;
+  }
   if (!g_nudge) return v;
   int dir = g_nudge;
   if (g_nudge == 2) {  // sign per call, from a hash of the value's bits
@@ -1654,6 +1668,10 @@ static bool build_chain(const mcv_comp_desc* comps, int ncomp, const double* off
 extern "C" {
 
 void orc_set_trans_nudge(int ulps) { orc::g_nudge = ulps; }
+void orc_set_pinned_nudge(int64_t nidx, const int32_t* dirs, int64_t ndirs) {
+  orc::g_pin_idx = nidx;
+  orc::g_pin_dir.assign(dirs, dirs + (ndirs > 0 ? ndirs : 0));
+}
 
 // ---- unit-level entry points (tests pin each against the reference's KATs)
 void orc_rng_states(uint64_t seed, int64_t n, uint64_t* out) {
@@ -1825,7 +1843,14 @@ int64_t orc_run(const mcv_comp_desc* comps, int ncomp, const double* offsets, co
       for (int64_t idx = start; idx < end; ++idx) {
         processed += 1;
         bool alive = true;
+#ifndef ORC_FLOP_BUILD
+        t_pin = idx == g_pin_idx;
+        t_call = 0;
+#endif
         int num = run_neutron(C, rng, n, h, msbuf, alive, idx);
+#ifndef ORC_FLOP_BUILD
+        t_pin = false;
+#endif
         if (dump_states && idx >= dump_first && idx < dump_first + dump_count) {
           int64_t k = idx - dump_first;
           std::memcpy(dump_states + 10 * k, n, 10 * sizeof(double));

@@ -64,6 +64,7 @@ def lib():
         L.orc_component.argtypes = [ctypes.POINTER(CompDesc), _dp, _up, _dp, _dp, ctypes.c_int]
         L.orc_partition.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _ip]
         L.orc_set_trans_nudge.argtypes = [ctypes.c_int]
+        L.orc_set_pinned_nudge.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
         L.orc_process.restype = ctypes.c_int64
         L.orc_process.argtypes = [ctypes.POINTER(CompDesc), ctypes.c_int, _dp, _dp, _dp, ctypes.c_int64,
                                   ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint32, _dp, _dp,
@@ -198,6 +199,14 @@ def partition(N, ntotalthreads=0, tpb=0):
 def set_trans_nudge(ulps):
     """Move every transcendental result by `ulps` (sign only) ulp: conditioning probe."""
     lib().orc_set_trans_nudge(int(ulps))
+
+
+def set_pinned_nudge(nidx=-1, dirs=()):
+    """For neutron `nidx` only, move its k-th transcendental result by dirs[k]
+    (-1, 0, +1) ulp; nidx=-1 clears it.  Reproduction probe: a device state this
+    matches bit for bit differs from the oracle's by exactly those ulps."""
+    d = np.ascontiguousarray(dirs, dtype=np.int32)
+    lib().orc_set_pinned_nudge(int(nidx), d.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(d.size))
 
 
 def run(descs, offsets, rotmats, N, seed, ntotalthreads=0, tpb=0, gate=True, tid_range=(0, -1), nthreads=0,
