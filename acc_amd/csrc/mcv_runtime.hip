@@ -636,14 +636,15 @@ struct CacheMergeArgs {
   const double* dump;
   const int32_t* doff;
   int32_t groups, slots, ntiles, nmon;
-  int32_t split;  // workgroups per tile, each taking kMergeGroups dumps
+  int32_t split;  // workgroups per tile, each taking `per` dumps
+  int32_t per;    // dumps per workgroup (<= 64)
   int32_t comp_base[MCV_MAX_COMP + 1];
   int64_t hoff[MCV_MAX_COMP], nbins[MCV_MAX_COMP];
 };
-constexpr int kMergeGroups = 8;   // dumps per merge workgroup (<= 64)
+constexpr int kMergeGroups = 8;   // dumps per merge workgroup of one launch's dumps
 constexpr int kMergeBatch = 4;    // entries a thread loads before adding them
 
-// kMergeGroups dumps per workgroup, `split` workgroups per tile of the cache's
+// `per` dumps per workgroup, `split` workgroups per tile of the cache's
 // key space: a workgroup lays its dumps' runs for its tile end to end (a
 // prefix sum over at most 64 runs), adds every entry into an LDS copy of the
 // tile (LDS FP64 atomics, kMergeBatch independent loads per thread in
@@ -655,14 +656,14 @@ __global__ __launch_bounds__(1024) void mcv_cache_merge(CacheMergeArgs M) {
   __shared__ double tile[3 * MCV_CACHE_TILE];
   __shared__ int run_lo[64], run_pre[65];
   const int t = (int)blockIdx.x / M.split, part = (int)blockIdx.x % M.split;
-  const int g0 = part * kMergeGroups;
+  const int g0 = part * M.per;
   int k = 0;
   while (k + 1 < M.nmon && M.comp_base[k + 1] <= t) ++k;
   for (int j = threadIdx.x; j < 3 * MCV_CACHE_TILE; j += blockDim.x) tile[j] = 0;
   if (threadIdx.x < 64) {
     const int lane = (int)threadIdx.x, g = g0 + lane;
     int lo = 0, n = 0;
-    if (lane < kMergeGroups && g < M.groups) {
+    if (lane < M.per && g < M.groups) {
       const int32_t* off = M.doff + (int64_t)g * (M.ntiles + 1);
       lo = off[t];
       n = off[t + 1] - lo;
@@ -884,6 +885,7 @@ struct mcv_instr {
   int64_t rep_total = 0;
   double* d_cdump = nullptr;  // bin-cache dumps of the resident workgroups
   int32_t* d_cdoff = nullptr;
+  int64_t cdump_sets = 0;  // launches' worth of dumps allocated side by side
   uint64_t* d_rng = nullptr;
   int64_t rng_cap = 0;
   unsigned long long* d_counter = nullptr;
@@ -1093,12 +1095,52 @@ int create_impl(const mcv_comp_desc* comps, int32_t ncomp, const double* offsets
     HIPCHECK(hipMalloc(&h->d_cdoff, (size_t)(groups * (h->args.cache_ntiles + 1)) * sizeof(int32_t)));
     h->args.cdump = h->d_cdump;
     h->args.cdoff = h->d_cdoff;
+    h->cdump_sets = 1;
   }
   return MCV_OK;
 }
 
-// Adds the bin-cache dumps of a launch's `groups` workgroups into the histograms.
-int merge_cache(mcv_instr* h, int64_t groups) {
+// A run in slices keeps the bin-cache dumps of up to kDumpSets launches side
+// by side (set s of a launch grid of g workgroups at groups [s g, (s + 1) g))
+// and merges them together: one merge per kDumpSets launches instead of one
+// per launch, each of which pays its own ramp over a few MB of dumps.
+constexpr int64_t kDumpSets = 16;
+constexpr int64_t kDumpBytes = int64_t(1) << 30;  // all sets of a handle
+
+// Sets for a run of `want` launches, within kDumpSets and kDumpBytes; grown
+// when `may_grow` (a failed allocation keeps the sets there are), else the
+// sets the handle has.
+int dump_sets(mcv_instr* h, int64_t want, bool may_grow, int64_t& sets) {
+  const int64_t groups = (int64_t)h->ctx->cus * std::max(h->per_cu, 1);
+  const int64_t dd = groups * cache_dump_doubles(h->args), dn = groups * (h->args.cache_ntiles + 1);
+  const int64_t set_bytes = dd * (int64_t)sizeof(double) + dn * (int64_t)sizeof(int32_t);
+  want = std::max<int64_t>(1, std::min({want, kDumpSets, kDumpBytes / std::max<int64_t>(set_bytes, 1)}));
+  if (may_grow && want > h->cdump_sets) {
+    double* d = nullptr;
+    int32_t* o = nullptr;
+    if (hipMalloc(&d, (size_t)(want * dd) * sizeof(double)) == hipSuccess &&
+        hipMalloc(&o, (size_t)(want * dn) * sizeof(int32_t)) == hipSuccess) {
+      HIPCHECK(hipFree(h->d_cdump));
+      HIPCHECK(hipFree(h->d_cdoff));
+      h->d_cdump = d;
+      h->d_cdoff = o;
+      h->args.cdump = d;
+      h->args.cdoff = o;
+      h->cdump_sets = want;
+    } else {
+      if (d) (void)hipFree(d);
+      (void)hipGetLastError();  // the run goes on with the sets there are
+    }
+  }
+  sets = std::max<int64_t>(1, std::min(want, h->cdump_sets));
+  return MCV_OK;
+}
+
+// Adds the bin-cache dumps of `groups` workgroups (one launch's, or several
+// launches' side by side) into the histograms.  Dumps of several launches go
+// up to 64 to a workgroup, so that the merge's workgroups stay a few per CU:
+// each zeroes and scans a whole tile, whatever its dumps hold.
+int merge_cache(mcv_instr* h, int64_t groups, bool sets = false) {
   const mcv::LaunchArgs& A = h->args;
   CacheMergeArgs M{};
   M.H = h->d_hist;
@@ -1115,7 +1157,11 @@ int merge_cache(mcv_instr* h, int64_t groups) {
     M.nmon += 1;
   }
   M.comp_base[M.nmon] = A.cache_ntiles;
-  M.split = (int32_t)std::max<int64_t>(1, (groups + kMergeGroups - 1) / kMergeGroups);
+  M.per = kMergeGroups;
+  if (sets)
+    M.per = (int32_t)std::min<int64_t>(64, std::max<int64_t>(kMergeGroups, (groups * A.cache_ntiles + 4 * h->ctx->cus - 1) /
+                                                                             (4 * (int64_t)h->ctx->cus)));
+  M.split = (int32_t)std::max<int64_t>(1, (groups + M.per - 1) / M.per);
   hipLaunchKernelGGL(mcv_cache_merge, dim3((unsigned)(A.cache_ntiles * M.split)), dim3(1024), 0, h->stream, M);
   HIPCHECK(hipGetLastError());
   return MCV_OK;
@@ -1185,7 +1231,7 @@ int64_t slice_per_launch(const mcv_instr* h, int64_t s_used, int64_t npt, uint64
 template <class States>
 int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads, int32_t tpb, uint32_t flags,
              uint64_t buffer_size, mcv_run_stats* stats, States states, unsigned long long* counter,
-             int64_t max_states = INT64_MAX) {
+             int64_t max_states = INT64_MAX, bool may_alloc = true) {
   const auto t0 = std::chrono::steady_clock::now();
   int64_t nblocks, nthreads, npt;
   partition(ncount, ntotalthreads, tpb, nblocks, nthreads, npt);
@@ -1214,6 +1260,9 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
   const unsigned lds = (unsigned)(lds_doubles(h->args) * sizeof(double));
   // with the bin cache, a grid of the resident workgroups, each running many stream blocks
   const int64_t cache_grid = h->args.cache_slots > 0 ? (int64_t)h->ctx->cus * std::max(h->per_cu, 1) : 0;
+  // slices: every launch has the same grid, so launches' dumps can share a merge
+  int64_t nsets = 1;
+  if (cache_grid > 0 && slice > 0 && nchunks > 1 && (rc = dump_sets(h, nchunks, may_alloc, nsets))) return rc;
   double ms_rng = 0, ms_k = 0;
   auto harvest = [&](int64_t c) -> int {
     hipEvent_t* e = h->rev + 3 * (c % kEvRing);
@@ -1263,10 +1312,18 @@ int run_impl(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t ntotalthreads
     int64_t grid = (s1 - s0 + per_group - 1) / per_group;
     if (h->pool) grid = std::min(grid, pool_grid);
     if (cache_grid > 0) grid = std::min(grid, cache_grid);
+    const int64_t set = c % nsets;
+    if (cache_grid > 0) {
+      A.cdump = h->d_cdump + set * grid * cache_dump_doubles(h->args);
+      A.cdoff = h->d_cdoff + set * grid * (h->args.cache_ntiles + 1);
+    }
     void* kargs[] = {&A};
     HIPCHECK(hipModuleLaunchKernel(h->fn, (unsigned)grid, 1, 1, (unsigned)h->block, 1, 1, lds, h->stream, kargs,
                                    nullptr));
-    if (cache_grid > 0 && (rc = merge_cache(h, grid))) return rc;  // before the next launch reuses the dumps
+    // the sets filled so far, before the next launch reuses the first
+    if (cache_grid > 0 && (set + 1 == nsets || c + 1 == nchunks) &&
+        (rc = merge_cache(h, grid * (set + 1), nsets > 1)))
+      return rc;
     if (c + 1 == nchunks && (rc = fold_replicas(h))) return rc;
     HIPCHECK(hipEventRecord(e[2], h->stream));
   }
@@ -1378,7 +1435,7 @@ int mcv_instr_run_noalloc(mcv_instr* h, uint64_t ncount, uint64_t seed, int64_t 
     return MCV_OK;
   };
   return run_impl(h, ncount, seed, ntotalthreads, tpb, flags, buffer_size, stats, states,
-                  reinterpret_cast<unsigned long long*>(base), kScratchStreams);
+                  reinterpret_cast<unsigned long long*>(base), kScratchStreams, false);
 }
 
 int mcv_monitor_read(mcv_instr* h, int32_t comp, double* out3) {
