@@ -28,7 +28,7 @@
 //    sloped-wall reflection: pinned by closed forms computed from the
 //    neutron's own state (tests/test_oracle_pins.py); the grid and DOS kernels
 //    and the detector have no reference counterpart, so those pins state what
-//    this build defines them to be.
+//    this engine defines them to be.
 
 #include <cmath>
 #include <cstdint>

@@ -15,8 +15,10 @@ for rel in LINES:
 FILE = re.compile(r"((?:[\w./-]+/)?[\w-]+\.(?:py|txt|yaml|yml|md|ipynb|xml|cfg))(?=[:\s,)`'\"]|$)")
 SPAN = re.compile(r":(\d+)(?:-(\d+))?((?:,\s*:?\d+(?:-\d+)?)*)")
 BARE = re.compile(r"(?<![\w./-])(?:\(|\s|,|`|^)(:\d+(?:-\d+)?)")
-# the build's own notes and the reference's survey files are not sources
-SKIP_TOP = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md", "BASELINE.json"}
+# At the top level, the notes checked are those that carry this line; other
+# top-level documents and records (surveys, reviews, run records) are not
+# this repository's sources.
+MARK = "<!-- reference citations checked by tests/test_citations.py -->"
 SKIP_DIRS = {"_work", "gpurun_out", "_kcache", "__pycache__", ".git", "golden", "lib", "profiles"}
 TEXT = (".py", ".h", ".hip", ".cpp", ".md", ".json", "Makefile")
 
@@ -25,8 +27,10 @@ def _sources():
     for dp, dn, fn in os.walk(ROOT):
         dn[:] = [d for d in dn if d not in SKIP_DIRS and not d.startswith(".")]
         for f in fn:
-            if dp == ROOT and (f in SKIP_TOP or f.startswith(("BENCH_", "GPUTEST_", "SCALE_", "MULTICHIP_"))):
-                continue
+            if dp == ROOT and not f.endswith(".py"):
+                with open(os.path.join(dp, f), errors="replace") as fh:
+                    if MARK not in fh.read():
+                        continue
             if f.endswith(TEXT):
                 yield os.path.join(dp, f)
     yield os.path.join(ROOT, "tests", "golden", "reference_kats.json")
