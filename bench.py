@@ -228,7 +228,8 @@ def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_n
     over a run's launches) and wall time per run at the reference partition,
     neutrons/s, F_alg and the FP64 fraction, and the CPU oracle over a bounded
     sample (all host cores; one core too with cpu_one_core).  c2 and c2s run
-    at their autotuned shape, c1 and c3 at the default one; c1 runs 1e7
+    at their autotuned shape, c1's one launch at the better of 512- and
+    1,024-lane workgroups, c1 in launches and c3 at the default one; c1 runs 1e7
     neutrons in launches of 1e6 as configured and "c1-one-launch" in one
     launch; "c2-ungated" keeps absorbed neutrons flowing (the reference's fused
     kernel exactly).  Rows of the same chain share its F_alg and CPU figures."""
@@ -247,6 +248,11 @@ def config_rows(names=None, steps=3, cpu_seconds=4.0, cpu_one_core=False, falg_n
         if chain in ("c2", "c2s"):
             h, best = tuned_handle(instr, ncount, use_cache=False)
             shape = "%s/%s/%s" % (best["block"], best["waves_per_eu"], best["streams_per_lane"])
+        elif name == "c1-one-launch":
+            # one launch gains from 1,024-lane workgroups (16 waves a CU, a few
+            # registers spilled); launches of 1e6 lose from them
+            h, best = tuned_handle(instr, ncount, blocks=(512, 1024), waves=(None,), use_cache=False)
+            shape = "%s/%s" % (best["block"], best["waves_per_eu"])
         else:
             h, shape = get_handle(instr), "default"
         h.clear()
